@@ -1,0 +1,184 @@
+// fft_lds.h -- LDS-resident Stockham FFT for one workgroup.
+//
+// A workgroup transforms `nrows` rows of length n that sit in LDS, element
+// (row, i) at s[row*rs + i*es].  Each stage of radix R reads its R inputs
+// (stride n/R) from one LDS buffer, applies the twiddles
+// exp(-2 pi i r k / (Ns R)), runs the R-point DFT in registers and writes
+// the outputs autosorted (Stockham) into the other buffer, so the result ends
+// in natural order with no bit reversal and one barrier per stage.  Keeping
+// only one butterfly in registers at a time (rather than all of a thread's
+// butterflies across a barrier) holds the kernels near 50 VGPRs (FP32) with
+// any mix of radices.  Twiddles come from a table tw[t] = exp(-2 pi i t / n).
+// Forward transforms only; an inverse is conj -> forward -> conj, which the
+// callers fold into their loads and stores.
+#pragma once
+
+#include "mmc_internal.h"
+
+namespace mmc {
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
+
+// ---- R-point forward DFTs (sign -1) in registers ---------------------------
+template <typename T, int R> struct Dft;
+
+template <typename T> struct Dft<T, 2> {
+    static __device__ __forceinline__ void run(cpx<T> *v)
+    {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <typename T> struct Dft<T, 3> {
+    static __device__ __forceinline__ void run(cpx<T> *v)
+    {
+        const T c = T(-0.5);
+        const T s = T(-0.86602540378443864676);   // -sin(2 pi / 3)
+        cpx<T> t1 = cadd(v[1], v[2]);
+        cpx<T> t2 = csub(v[1], v[2]);
+        cpx<T> m = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+        v[0] = cadd(v[0], t1);
+        // +- i s t2
+        v[1] = {m.x - s * t2.y, m.y + s * t2.x};
+        v[2] = {m.x + s * t2.y, m.y - s * t2.x};
+    }
+};
+
+template <typename T> struct Dft<T, 4> {
+    static __device__ __forceinline__ void run(cpx<T> *v)
+    {
+        cpx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        cpx<T> c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+        v[0] = cadd(a, c);
+        v[2] = csub(a, c);
+        v[1] = {b.x + d.y, b.y - d.x};   // b - i d
+        v[3] = {b.x - d.y, b.y + d.x};   // b + i d
+    }
+};
+
+template <typename T> struct Dft<T, 5> {
+    static __device__ __forceinline__ void run(cpx<T> *v)
+    {
+        const T c1 = T(0.30901699437494742410);    // cos(2 pi/5)
+        const T c2 = T(-0.80901699437494742410);   // cos(4 pi/5)
+        const T s1 = T(-0.95105651629515357212);   // -sin(2 pi/5)
+        const T s2 = T(-0.58778525229247312917);   // -sin(4 pi/5)
+        cpx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        cpx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        cpx<T> x0 = v[0];
+        cpx<T> r1 = {x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+        cpx<T> r2 = {x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+        cpx<T> i1 = {s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y};
+        cpx<T> i2 = {s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y};
+        v[0] = {x0.x + a1.x + a2.x, x0.y + a1.y + a2.y};
+        v[1] = {r1.x - i1.y, r1.y + i1.x};   // r1 + i*i1
+        v[4] = {r1.x + i1.y, r1.y - i1.x};   // r1 - i*i1
+        v[2] = {r2.x - i2.y, r2.y + i2.x};
+        v[3] = {r2.x + i2.y, r2.y - i2.x};
+    }
+};
+
+template <typename T> struct Dft<T, 7> {
+    static __device__ __forceinline__ void run(cpx<T> *v)
+    {
+        const T c[4] = {T(1.0), T(0.62348980185873353053), T(-0.22252093395631440429),
+                        T(-0.90096886790241912624)};
+        const T s[4] = {T(0.0), T(-0.78183148246802980871), T(-0.97492791218182360702),
+                        T(-0.43388373911755812048)};
+        cpx<T> a[4], b[4];
+        #pragma unroll
+        for (int r = 1; r < 4; ++r) { a[r] = cadd(v[r], v[7 - r]); b[r] = csub(v[r], v[7 - r]); }
+        cpx<T> x0 = v[0];
+        cpx<T> out[7];
+        out[0] = {x0.x + a[1].x + a[2].x + a[3].x, x0.y + a[1].y + a[2].y + a[3].y};
+        #pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            cpx<T> re = x0, im = {T(0), T(0)};
+            #pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                const int e = (r * k) % 7;
+                const int q = e < 4 ? e : 7 - e;
+                const T cs = c[q];
+                const T sn = e < 4 ? s[q] : -s[q];
+                re.x += cs * a[r].x; re.y += cs * a[r].y;
+                im.x += sn * b[r].x; im.y += sn * b[r].y;
+            }
+            out[k] = {re.x - im.y, re.y + im.x};
+            out[7 - k] = {re.x + im.y, re.y - im.x};
+        }
+        #pragma unroll
+        for (int r = 0; r < 7; ++r) v[r] = out[r];
+    }
+};
+
+// ---- one Stockham stage: in -> out ---------------------------------------------
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(const cpx<T> *__restrict__ in,
+                                          cpx<T> *__restrict__ out, int n, int nrows,
+                                          int es, int rs, int Ns,
+                                          const cpx<T> *__restrict__ tw)
+{
+    const int nb = n / R;
+    const int total = nb * nrows;
+    const int tws = n / (Ns * R);
+    for (int t = threadIdx.x; t < total; t += kThreads) {
+        const int row = t / nb;
+        const int j = t - row * nb;
+        const int k = j % Ns;
+        const cpx<T> *ib = in + row * rs;
+        cpx<T> v[R];
+        #pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = ib[(j + r * nb) * es];
+        if (Ns > 1) {
+            #pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * k * tws]);
+        }
+        Dft<T, R>::run(v);
+        cpx<T> *ob = out + row * rs;
+        const int o = (j - k) * R + k;
+        #pragma unroll
+        for (int r = 0; r < R; ++r) ob[(o + r * Ns) * es] = v[r];
+    }
+    __syncthreads();
+}
+
+// Transform nrows rows held in s0 with plan pl, ping-ponging with s1 (same
+// size and layout).  Returns the buffer holding the result.  The caller must
+// have synchronised after writing s0; the result is synchronised on return.
+template <typename T>
+__device__ cpx<T> *fft_rows(cpx<T> *s0, cpx<T> *s1, int nrows, int es, int rs,
+                            const FftPlan &pl, const cpx<T> *__restrict__ tw)
+{
+    const int n = pl.n;
+    const int nst = pl.nst;
+    int Ns = 1;
+    for (int st = 0; st < nst; ++st) {
+        const int R = pl.radix[st];
+        switch (R) {
+        case 4: fft_stage<T, 4>(s0, s1, n, nrows, es, rs, Ns, tw); break;
+        case 2: fft_stage<T, 2>(s0, s1, n, nrows, es, rs, Ns, tw); break;
+        case 3: fft_stage<T, 3>(s0, s1, n, nrows, es, rs, Ns, tw); break;
+        case 5: fft_stage<T, 5>(s0, s1, n, nrows, es, rs, Ns, tw); break;
+        default: fft_stage<T, 7>(s0, s1, n, nrows, es, rs, Ns, tw); break;
+        }
+        cpx<T> *t = s0;
+        s0 = s1;
+        s1 = t;
+        Ns *= R;
+    }
+    return s0;
+}
+
+}  // namespace mmc

@@ -1,0 +1,1062 @@
+// mmc_capi.hip -- context, device memory, work lists and the C ABI.
+//
+// Host-side restatement of the reference's set-up and control flow
+// (src/main.cu:254-756 allocations, padding, tensor + cache, FFT plans;
+// :801-1139 the anneal loop), driving the kernels of mmc_kernels.hip on one
+// HIP stream per context.
+#include "mmc.h"
+#include "mmc_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+using namespace mmc;
+
+namespace {
+
+constexpr double kPiTrunc = 3.14159265358;       // src/constant.h:21
+constexpr double kBoltzmann = 1.3806488E-23;     // src/constant.h:22
+
+const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy"};
+constexpr int kNumKinds = 8;
+enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY };
+
+struct List {
+    std::vector<Work> host;
+    std::vector<int> first;   // first item of each instance, size count+1
+    Work *dev = nullptr;
+    size_t lds = 0;
+    int n() const { return (int)host.size(); }
+};
+
+bool is_smooth(int v)
+{
+    for (int p : {2, 3, 5}) while (v % p == 0) v /= p;
+    return v == 1;
+}
+
+int padded_len(int n, int pbc)                  // src/main.cu:254-263
+{
+    if (pbc != 0) return n;
+    int v = std::max(1, 2 * n - 1);
+    while (!is_smooth(v)) ++v;
+    return v;
+}
+
+bool make_plan(int n, FftPlan &pl)
+{
+    pl.n = n;
+    pl.nst = 0;
+    int v = n;
+    while (v % 4 == 0) { pl.radix[pl.nst++] = 4; v /= 4; }
+    for (int p : {2, 3, 5, 7})
+        while (v % p == 0) {
+            if (pl.nst >= kMaxStages) return false;
+            pl.radix[pl.nst++] = p;
+            v /= p;
+        }
+    return v == 1;
+}
+
+}  // namespace
+
+struct mmc_ctx {
+    int precision = MMC_FP64;
+    int count = 0;
+    size_t tsize = 8;                 // sizeof(T)
+    std::vector<mmc_params> params;
+    std::vector<Inst> inst;
+    Inst *dInst = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    size_t lds_pref = 64 * 1024;      // preferred LDS per FFT workgroup
+    size_t lds_max = 64 * 1024;       // device limit per workgroup
+
+    // pools
+    void *d_m = nullptr, *d_h = nullptr, *d_a = nullptr, *d_b = nullptr, *d_khat = nullptr;
+    double *d_K = nullptr;
+    void *d_tw32 = nullptr, *d_tw64 = nullptr;
+    double *d_beta = nullptr;
+    uint32_t *d_ctl = nullptr;
+    double *d_part = nullptr, *d_eout = nullptr;
+    int *d_efirst = nullptr;
+    double *d_H = nullptr;            // field output staging (largest instance)
+    size_t m_elems = 0, a_elems = 0, b_elems = 0, khat_elems = 0, k_elems = 0, ka_elems = 0;
+    long maxN = 0;
+    bool b_alias = true;              // every instance has Pz == 1: B is A
+
+    List xl, zl, yl, cl, tl, txl, tzl, tyl;   // magnetisation passes, cells, tensor passes
+    int max_colors = 2;
+    bool any_demag = false;
+    bool tensor_ready = false;
+    std::vector<double> T;
+
+    // options
+    bool use_graph = false;
+    bool timing = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_sweeps = 0;
+
+    struct Rec { int kind; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    long launches[kNumKinds] = {0};
+    double total_ms[kNumKinds] = {0};
+
+    std::string err;
+};
+
+namespace {
+
+int fail(mmc_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                               \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail((c), MMC_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- timing ---------------------------------------------------------------
+hipEvent_t next_event(mmc_ctx *c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+void harvest(mmc_ctx *c)
+{
+    if (c->recs.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto &r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->launches[r.kind]++;
+            c->total_ms[r.kind] += ms;
+        }
+    }
+    c->recs.clear();
+    c->ev_used = 0;
+}
+
+template <typename F>
+hipError_t timed(mmc_ctx *c, int kind, F &&f)
+{
+    if (!c->timing) return f();
+    if (c->ev_used + 2 > 4096) harvest(c);
+    hipEvent_t a = next_event(c), b = next_event(c);
+    if (!a || !b) return f();
+    hipEventRecord(a, c->stream);
+    hipError_t e = f();
+    hipEventRecord(b, c->stream);
+    c->recs.push_back({kind, a, b});
+    return e;
+}
+
+// ---- tiles ----------------------------------------------------------------
+// Largest unit count <= want whose two ping-pong LDS buffers fit the
+// preferred per-workgroup budget (two workgroups per CU), else one unit if
+// it fits the device maximum, else 0.
+int fit_units(int want, size_t bytes_per_unit, size_t pref, size_t maxb)
+{
+    int u = want;
+    while (u > 1 && (size_t)u * bytes_per_unit > pref) --u;
+    if ((size_t)u * bytes_per_unit <= maxb) return u;
+    return 0;
+}
+
+bool choose_tiles(const mmc_ctx *c, Inst &in, size_t pref, size_t maxb)
+{
+    const size_t cs = 2 * c->tsize;
+    const int tymax = c->tsize == 4 ? 16 : 8;
+    const int pairs = fit_units(tymax / 2, 2 * (size_t)in.Px * cs, pref, maxb);
+    in.tyx = 2 * pairs;
+    in.tyz = fit_units(tymax, 2 * (size_t)in.Pz * cs, pref, maxb);
+    in.ncol = fit_units(4, 2 * 3 * (size_t)in.Py * cs, pref, maxb);
+    in.tyx64 = 2 * fit_units(4, 2 * (size_t)in.Px * 16, pref, maxb);
+    in.tyz64 = fit_units(8, 2 * (size_t)in.Pz * 16, pref, maxb);
+    const bool yt = 2 * (size_t)in.Py * 16 <= maxb;
+    return pairs && in.tyz && in.ncol && in.tyx64 && in.tyz64 && yt;
+}
+
+int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
+{
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1)
+        return fail(c, MMC_EINVAL, "instance %d: grid must be at least 1x1x1", idx);
+    if (!(p.dx > 0 && p.dy > 0 && p.dz > 0))
+        return fail(c, MMC_EINVAL, "instance %d: cell size must be positive", idx);
+    if (!(p.ms > 0))
+        return fail(c, MMC_EINVAL, "instance %d: ms must be positive", idx);
+    if ((long)p.nx * p.ny * p.nz > (1L << 31) - 1)
+        return fail(c, MMC_EINVAL, "instance %d: more than 2^31-1 cells", idx);
+    std::memset(&in, 0, sizeof(in));
+    in.nx = p.nx; in.ny = p.ny; in.nz = p.nz;
+    in.pbc[0] = p.pbc_x != 0; in.pbc[1] = p.pbc_y != 0; in.pbc[2] = p.pbc_z != 0;
+    in.img[0] = std::max(p.pbc_x, 0);           // src/main.cu:475-477
+    in.img[1] = std::max(p.pbc_y, 0);
+    in.img[2] = std::max(p.pbc_z, 0);
+    in.Px = padded_len(p.nx, p.pbc_x);
+    in.Py = padded_len(p.ny, p.pbc_y);
+    in.Pz = padded_len(p.nz, p.pbc_z);
+    in.Hx = in.Px / 2 + 1;
+    in.N = (long)p.nx * p.ny * p.nz;
+    in.NP = (long)in.Px * in.Py * in.Pz;
+    if (!make_plan(in.Px, in.plx) || !make_plan(in.Py, in.ply) || !make_plan(in.Pz, in.plz))
+        return fail(c, MMC_EUNSUP,
+                    "instance %d: padded lengths (%d,%d,%d) must factor into 2,3,5,7 "
+                    "(a periodic axis keeps its own length)", idx, in.Px, in.Py, in.Pz);
+    if (!choose_tiles(c, in, c->lds_pref, c->lds_max))
+        return fail(c, MMC_EUNSUP,
+                    "instance %d: padded grid (%d,%d,%d) does not fit the %zu-byte LDS "
+                    "per workgroup", idx, in.Px, in.Py, in.Pz, c->lds_max);
+    in.ncolors = 2;
+    for (int a = 0; a < 3; ++a) {
+        int n = a == 0 ? p.nx : (a == 1 ? p.ny : p.nz);
+        if (in.pbc[a] && n >= 3 && (n & 1)) in.ncolors = 3;
+    }
+    in.cal_demag = p.cal_demag ? 1 : 0;
+    // physics in reduced units (fields per Ms, energies per mu0 Ms^2 V)
+    const double mu0 = 4e-7 * kPiTrunc;          // src/main.cu:244
+    const double ms = p.ms;
+    in.d[0] = p.dx; in.d[1] = p.dy; in.d[2] = p.dz;
+    const double dd[3] = {p.dx, p.dy, p.dz};
+    for (int a = 0; a < 3; ++a) in.lam[a] = 2.0 * p.aex / (mu0 * ms * ms * dd[a] * dd[a]);
+    for (int a = 0; a < 2; ++a) in.dmi[a] = p.dind / (mu0 * ms * ms * dd[a]);
+    in.kap = p.ku1 / (mu0 * ms * ms);
+    const double ul = std::sqrt(p.anis_ux * p.anis_ux + p.anis_uy * p.anis_uy + p.anis_uz * p.anis_uz);
+    if (ul > 0) { in.u[0] = p.anis_ux / ul; in.u[1] = p.anis_uy / ul; in.u[2] = p.anis_uz / ul; }
+    in.b[0] = p.bext_x / (mu0 * ms);
+    in.b[1] = p.bext_y / (mu0 * ms);
+    in.b[2] = p.bext_z / (mu0 * ms);
+    in.ms = ms;
+    in.e_unit = mu0 * ms * ms * p.dx * p.dy * p.dz;
+    in.key[0] = (uint32_t)p.seed;
+    in.key[1] = (uint32_t)(p.seed >> 32);
+    return MMC_OK;
+}
+
+int upload_list(mmc_ctx *c, List &l)
+{
+    if (l.dev) { hipFree(l.dev); l.dev = nullptr; }
+    if (l.host.empty()) return MMC_OK;
+    HIPCHK(c, hipMalloc(&l.dev, sizeof(Work) * l.host.size()));
+    HIPCHK(c, hipMemcpy(l.dev, l.host.data(), sizeof(Work) * l.host.size(), hipMemcpyHostToDevice));
+    return MMC_OK;
+}
+
+void build_lists(mmc_ctx *c)
+{
+    const size_t cs = 2 * c->tsize;   // complex element size
+    auto reset = [&](List &l) { l.host.clear(); l.first.assign(1, 0); l.lds = 0; };
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
+    for (int i = 0; i < c->count; ++i) {
+        const Inst &in = c->inst[i];
+        if (in.cal_demag) {
+            for (int comp = 0; comp < 3; ++comp)
+                for (int z = 0; z < in.nz; ++z)
+                    for (int y0 = 0; y0 < in.ny; y0 += in.tyx) c->xl.host.push_back({i, comp, z, y0});
+            c->xl.lds = std::max(c->xl.lds, 2 * (size_t)(in.tyx / 2) * in.Px * cs);
+            if (in.Pz > 1) {
+                for (int comp = 0; comp < 3; ++comp)
+                    for (int kx = 0; kx < in.Hx; ++kx)
+                        for (int y0 = 0; y0 < in.ny; y0 += in.tyz) c->zl.host.push_back({i, comp, kx, y0});
+                c->zl.lds = std::max(c->zl.lds, 2 * (size_t)in.Pz * in.tyz * cs);
+            }
+            for (int kz = 0; kz < in.Pz; ++kz)
+                for (int kx0 = 0; kx0 < in.Hx; kx0 += in.ncol)
+                    c->yl.host.push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
+            c->yl.lds = std::max(c->yl.lds, 2 * (size_t)3 * in.ncol * in.Py * cs);
+            // tensor passes (FP64)
+            for (int comp = 0; comp < 6; ++comp)
+                for (int z = 0; z < in.Pz; ++z)
+                    for (int y0 = 0; y0 < in.Py; y0 += in.tyx64) c->txl.host.push_back({i, comp, z, y0});
+            c->txl.lds = std::max(c->txl.lds, 2 * (size_t)(in.tyx64 / 2) * in.Px * 16);
+            if (in.Pz > 1) {
+                for (int comp = 0; comp < 6; ++comp)
+                    for (int kx = 0; kx < in.Hx; ++kx)
+                        for (int y0 = 0; y0 < in.Py; y0 += in.tyz64) c->tzl.host.push_back({i, comp, kx, y0});
+                c->tzl.lds = std::max(c->tzl.lds, 2 * (size_t)in.Pz * in.tyz64 * 16);
+            }
+            for (int comp = 0; comp < 6; ++comp)
+                for (int kx = 0; kx < in.Hx; ++kx)
+                    for (int kz = 0; kz < in.Pz; ++kz) c->tyl.host.push_back({i, comp, kx, kz});
+            c->tyl.lds = std::max(c->tyl.lds, 2 * (size_t)in.Py * 16);
+            for (long s = 0; s < in.NP; s += kThreads) c->tl.host.push_back({i, (int)s, 0, 0});
+        }
+        for (long s = 0; s < in.N; s += kThreads) c->cl.host.push_back({i, (int)s, 0, 0});
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl})
+            l->first.push_back(l->n());
+    }
+}
+
+template <typename T>
+void twiddles(int n, std::vector<cpx<T>> &out)
+{
+    for (int t = 0; t < n; ++t) {
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+        out.push_back({(T)cosl(a), (T)sinl(a)});
+    }
+}
+
+int alloc_all(mmc_ctx *c)
+{
+    const size_t ts = c->tsize, cs = 2 * ts;
+    size_t m = 0, a = 0, b = 0, kh = 0, k = 0, ka = 0, tw = 0;
+    std::vector<cpx<float>> tw32;
+    std::vector<cpx<double>> tw64;
+    c->b_alias = true;
+    for (auto &in : c->inst) if (in.cal_demag && in.Pz > 1) c->b_alias = false;
+    for (auto &in : c->inst) {
+        in.m_off = (long)m;
+        in.h_off = (long)m;
+        m += 3 * in.N;
+        c->maxN = std::max(c->maxN, in.N);
+        if (in.cal_demag) {
+            in.a_off = (long)a;
+            a += (size_t)3 * in.Hx * in.nz * in.ny;
+            if (!c->b_alias) {
+                in.b_off = (long)b;
+                b += (size_t)3 * in.Hx * in.Pz * in.ny;
+            } else {
+                in.b_off = in.a_off;
+            }
+            in.khat_off = (long)kh;
+            kh += (size_t)6 * in.Hx * in.Pz * in.Py;
+            in.k_off = (long)k;
+            k += (size_t)6 * in.NP;
+            in.ka_off = (long)ka;
+            ka += (size_t)6 * in.Hx * in.Pz * in.Py;
+            in.twx = (long)tw; twiddles(in.Px, tw32); twiddles(in.Px, tw64); tw += in.Px;
+            in.twy = (long)tw; twiddles(in.Py, tw32); twiddles(in.Py, tw64); tw += in.Py;
+            in.twz = (long)tw; twiddles(in.Pz, tw32); twiddles(in.Pz, tw64); tw += in.Pz;
+        }
+    }
+    c->m_elems = m; c->a_elems = a; c->b_elems = b; c->khat_elems = kh; c->k_elems = k;
+    c->ka_elems = ka;
+    HIPCHK(c, hipMalloc(&c->d_m, std::max<size_t>(1, m) * ts));
+    HIPCHK(c, hipMalloc(&c->d_h, std::max<size_t>(1, m) * ts));
+    HIPCHK(c, hipMemsetAsync(c->d_h, 0, std::max<size_t>(1, m) * ts, c->stream));
+    if (a) HIPCHK(c, hipMalloc(&c->d_a, a * cs));
+    if (b) HIPCHK(c, hipMalloc(&c->d_b, b * cs));
+    if (c->b_alias) c->d_b = c->d_a;
+    if (kh) HIPCHK(c, hipMalloc(&c->d_khat, kh * cs));
+    if (k) {
+        HIPCHK(c, hipMalloc(&c->d_K, k * sizeof(double)));
+        HIPCHK(c, hipMemsetAsync(c->d_K, 0, k * sizeof(double), c->stream));
+    }
+    if (tw) {
+        HIPCHK(c, hipMalloc(&c->d_tw32, tw * sizeof(cpx<float>)));
+        HIPCHK(c, hipMalloc(&c->d_tw64, tw * sizeof(cpx<double>)));
+        HIPCHK(c, hipMemcpy(c->d_tw32, tw32.data(), tw * sizeof(cpx<float>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_tw64, tw64.data(), tw * sizeof(cpx<double>), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMalloc(&c->d_beta, sizeof(double) * c->count));
+    HIPCHK(c, hipMalloc(&c->d_ctl, sizeof(uint32_t) * 4));
+    HIPCHK(c, hipMemsetAsync(c->d_ctl, 0, sizeof(uint32_t) * 4, c->stream));
+    HIPCHK(c, hipMalloc(&c->d_part, sizeof(double) * 16 * std::max(1, c->cl.n())));
+    HIPCHK(c, hipMalloc(&c->d_eout, sizeof(double) * 8 * c->count));
+    HIPCHK(c, hipMalloc(&c->d_efirst, sizeof(int) * (c->count + 1)));
+    HIPCHK(c, hipMemcpy(c->d_efirst, c->cl.first.data(), sizeof(int) * (c->count + 1),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
+    HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
+    HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+        int rc = upload_list(c, *l);
+        if (rc) return rc;
+    }
+    c->T.assign(c->count, 300.0);
+    std::vector<double> beta(c->count);
+    for (int i = 0; i < c->count; ++i) beta[i] = c->inst[i].e_unit / (kBoltzmann * c->T[i]);
+    HIPCHK(c, hipMemcpy(c->d_beta, beta.data(), sizeof(double) * c->count, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MMC_OK;
+}
+
+int create_common(const mmc_params *p, int count, int precision, mmc_ctx **out)
+{
+    if (!out) return MMC_EINVAL;
+    *out = nullptr;
+    if (!p || count < 1) return MMC_EINVAL;
+    if (precision != MMC_FP64 && precision != MMC_FP32) return MMC_EINVAL;
+    mmc_ctx *c = new mmc_ctx();
+    c->precision = precision;
+    c->tsize = precision == MMC_FP32 ? 4 : 8;
+    c->count = count;
+    c->params.assign(p, p + count);
+    c->inst.resize(count);
+    int rc;
+    if (hipGetDevice(&c->device) != hipSuccess) { *out = c; return fail(c, MMC_EHIP, "no HIP device"); }
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess &&
+            v > 0) {
+            c->lds_max = (size_t)v;
+            if (c->lds_max > 64 * 1024 && set_lds_limit(c->lds_max) != hipSuccess) c->lds_max = 64 * 1024;
+        }
+        c->lds_pref = std::min(c->lds_pref, c->lds_max);
+    }
+    for (int i = 0; i < count; ++i) {
+        if ((rc = setup_inst(c, i, p[i], c->inst[i]))) { *out = c; return rc; }
+        c->max_colors = std::max(c->max_colors, c->inst[i].ncolors);
+        if (c->inst[i].cal_demag) c->any_demag = true;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        *out = c;
+        return fail(c, MMC_EHIP, "hipStreamCreate failed");
+    }
+    build_lists(c);
+    rc = alloc_all(c);
+    *out = c;
+    return rc;
+}
+
+// ---- demag chain -------------------------------------------------------------
+template <typename T>
+int demag_chain(mmc_ctx *c)
+{
+    if (!c->any_demag) return MMC_OK;
+    if (!c->tensor_ready) return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
+    hipStream_t st = c->stream;
+    const cpx<T> *tw = (const cpx<T> *)(c->tsize == 4 ? c->d_tw32 : c->d_tw64);
+    T *m = (T *)c->d_m;
+    T *h = (T *)c->d_h;
+    cpx<T> *A = (cpx<T> *)c->d_a;
+    cpx<T> *B = (cpx<T> *)c->d_b;
+    const cpx<T> *Kh = (const cpx<T> *)c->d_khat;
+    HIPCHK(c, timed(c, K_XFWD, [&] { return launch_xfwd<T>(st, c->dInst, c->xl.dev, c->xl.n(), 0, m, A, tw, c->xl.lds); }));
+    if (c->zl.n())
+        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(st, c->dInst, c->zl.dev, c->zl.n(), 0, A, B, tw, c->zl.lds); }));
+    HIPCHK(c, timed(c, K_YMUL, [&] { return launch_ymul<T>(st, c->dInst, c->yl.dev, c->yl.n(), B, Kh, tw, c->yl.lds); }));
+    if (c->zl.n())
+        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, c->zl.dev, c->zl.n(), B, A, tw, c->zl.lds); }));
+    HIPCHK(c, timed(c, K_XINV, [&] { return launch_xinv<T>(st, c->dInst, c->xl.dev, c->xl.n(), A, h, tw, c->xl.lds); }));
+    return MMC_OK;
+}
+
+template <typename T>
+int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
+{
+    int rc = demag_chain<T>(c);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    for (int col = 0; col < c->max_colors; ++col)
+        HIPCHK(c, timed(c, K_METRO, [&] {
+                   return launch_metro<T>(st, c->dInst, c->cl.dev, c->cl.n(), (T *)c->d_m,
+                                          (const T *)c->d_h, c->d_beta, c->d_ctl, sweep_off, col,
+                                          c->any_demag ? 1 : 0);
+               }));
+    return MMC_OK;
+}
+
+template <typename T>
+int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
+{
+    hipStream_t st = c->stream;
+    if (c->any_demag && !c->tensor_ready)
+        return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
+    if (!c->use_graph || c->timing) {
+        HIPCHK(c, launch_set_ctl(st, c->d_ctl, stage, sweep0));
+        for (uint32_t s = 0; s < count; ++s) {
+            int rc = enqueue_sweep<T>(c, s);
+            if (rc) return rc;
+        }
+        return MMC_OK;
+    }
+    const int G = c->graph_sweeps > 0 ? c->graph_sweeps : 16;
+    if (!c->gexec) {
+        HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = MMC_OK;
+        for (int s = 0; s < G && rc == MMC_OK; ++s) rc = enqueue_sweep<T>(c, (uint32_t)s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) { if (g) hipGraphDestroy(g); return rc; }
+        HIPCHK(c, e);
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->graph_sweeps = G;
+    }
+    uint32_t s = 0;
+    for (; s + G <= count; s += G) {
+        HIPCHK(c, launch_set_ctl(st, c->d_ctl, stage, sweep0 + s));
+        HIPCHK(c, hipGraphLaunch(c->gexec, st));
+    }
+    if (s < count) {
+        HIPCHK(c, launch_set_ctl(st, c->d_ctl, stage, sweep0 + s));
+        for (uint32_t k = 0; s + k < count; ++k) {
+            int rc = enqueue_sweep<T>(c, k);
+            if (rc) return rc;
+        }
+    }
+    return MMC_OK;
+}
+
+// ---- tensor build --------------------------------------------------------
+std::string cache_name(const Inst &in, const mmc_params &p)
+{
+    // src/main.cu:421-423 with the reference's padded sizes (2n-1 on open axes)
+    const int rx = p.pbc_x ? p.nx : 2 * p.nx - 1;
+    const int ry = p.pbc_y ? p.ny : 2 * p.ny - 1;
+    const int rz = p.pbc_z ? p.nz : 2 * p.nz - 1;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "cache_%d_%d_%d-%g_%g_%g-%d_%d_%d.ovf", rx, ry, rz, p.dx, p.dy,
+             p.dz, p.pbc_x, p.pbc_y, p.pbc_z);
+    (void)in;
+    return buf;
+}
+
+// Map reference cache index (z fastest, reference padding) <-> our index.
+int ref_to_ours(int iref, int n, int Pref, int P, int periodic)
+{
+    if (periodic) return iref;
+    const int off = (iref + (Pref + 1) / 2 - 1) % Pref - (Pref + 1) / 2 + 1;  // src/demag_cu.cu:149
+    return (off % P + P) % P;
+}
+
+bool load_cache(const std::string &path, const Inst &in, const mmc_params &p, std::vector<double> &K)
+{
+    const int Pref[3] = {p.pbc_x ? p.nx : 2 * p.nx - 1, p.pbc_y ? p.ny : 2 * p.ny - 1,
+                         p.pbc_z ? p.nz : 2 * p.nz - 1};
+    const long NR = (long)Pref[0] * Pref[1] * Pref[2];
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.seekg(0, std::ios::end);
+    if ((long)f.tellg() != 6 * NR * 8) return false;
+    f.seekg(0);
+    std::vector<double> raw(6 * NR);
+    f.read(reinterpret_cast<char *>(raw.data()), 6 * NR * 8);
+    if (!f) return false;
+    K.assign(6 * in.NP, 0.0);
+    for (int comp = 0; comp < 6; ++comp)
+        for (int xr = 0; xr < Pref[0]; ++xr)
+            for (int yr = 0; yr < Pref[1]; ++yr)
+                for (int zr = 0; zr < Pref[2]; ++zr) {
+                    const long ridx = zr + (long)yr * Pref[2] + (long)xr * Pref[2] * Pref[1];
+                    const int x = ref_to_ours(xr, p.nx, Pref[0], in.Px, in.pbc[0]);
+                    const int y = ref_to_ours(yr, p.ny, Pref[1], in.Py, in.pbc[1]);
+                    const int z = ref_to_ours(zr, p.nz, Pref[2], in.Pz, in.pbc[2]);
+                    K[comp * in.NP + x + (long)in.Px * (y + (long)in.Py * z)] = raw[comp * NR + ridx];
+                }
+    return true;
+}
+
+bool save_cache(const std::string &path, const Inst &in, const mmc_params &p, const std::vector<double> &K)
+{
+    const int Pref[3] = {p.pbc_x ? p.nx : 2 * p.nx - 1, p.pbc_y ? p.ny : 2 * p.ny - 1,
+                         p.pbc_z ? p.nz : 2 * p.nz - 1};
+    const long NR = (long)Pref[0] * Pref[1] * Pref[2];
+    std::vector<double> raw(6 * NR);
+    for (int comp = 0; comp < 6; ++comp)
+        for (int xr = 0; xr < Pref[0]; ++xr)
+            for (int yr = 0; yr < Pref[1]; ++yr)
+                for (int zr = 0; zr < Pref[2]; ++zr) {
+                    const long ridx = zr + (long)yr * Pref[2] + (long)xr * Pref[2] * Pref[1];
+                    const int x = ref_to_ours(xr, p.nx, Pref[0], in.Px, in.pbc[0]);
+                    const int y = ref_to_ours(yr, p.ny, Pref[1], in.Py, in.pbc[1]);
+                    const int z = ref_to_ours(zr, p.nz, Pref[2], in.Pz, in.pbc[2]);
+                    raw[comp * NR + ridx] = K[comp * in.NP + x + (long)in.Px * (y + (long)in.Py * z)];
+                }
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.write(reinterpret_cast<const char *>(raw.data()), 6 * NR * 8);
+    return (bool)f;
+}
+
+// Tensor FFT (FP64) for all instances with demag: K -> KA -> Khat.
+template <typename T>
+int tensor_fft(mmc_ctx *c)
+{
+    hipStream_t st = c->stream;
+    cpx<double> *KA = nullptr;
+    HIPCHK(c, hipMalloc(&KA, std::max<size_t>(1, c->ka_elems) * sizeof(cpx<double>)));
+    const cpx<double> *tw = (const cpx<double> *)c->d_tw64;
+    int rc = MMC_OK;
+    hipError_t e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, c->d_K, KA, tw, c->txl.lds);
+    if (e == hipSuccess && c->tzl.n())
+        e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, KA, KA, tw, c->tzl.lds);
+    if (e == hipSuccess)
+        e = launch_yfwd_tensor<T>(st, c->dInst, c->tyl.dev, c->tyl.n(), KA, (cpx<T> *)c->d_khat, tw,
+                                  c->tyl.lds);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail(c, MMC_EHIP, "tensor FFT failed: %s", hipGetErrorString(e));
+    hipFree(KA);
+    return rc;
+}
+
+int refresh_mirror(mmc_ctx *c)
+{
+    for (int i = 0; i < c->count; ++i) {
+        Inst &in = c->inst[i];
+        if (!in.cal_demag) continue;
+        for (int k = 0; k < 6; ++k)
+            HIPCHK(c, hipMemcpy(&in.K0[k], c->d_K + in.k_off + k * in.NP, sizeof(double),
+                                hipMemcpyDeviceToHost));
+    }
+    HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
+    return MMC_OK;
+}
+
+int finish_tensor(mmc_ctx *c)
+{
+    int rc = refresh_mirror(c);
+    if (rc) return rc;
+    rc = c->tsize == 4 ? tensor_fft<float>(c) : tensor_fft<double>(c);
+    if (rc) return rc;
+    c->tensor_ready = true;
+    return MMC_OK;
+}
+
+int check_inst(const mmc_ctx *c, int inst)
+{
+    if (!c) return MMC_EINVAL;
+    if (inst < 0 || inst >= c->count) return fail(const_cast<mmc_ctx *>(c), MMC_EINVAL, "instance %d out of range", inst);
+    return MMC_OK;
+}
+
+template <typename T>
+int energy_impl(mmc_ctx *c, std::vector<double> &sums)
+{
+    int rc = demag_chain<T>(c);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    HIPCHK(c, timed(c, K_ENERGY, [&] {
+               return launch_energy<T>(st, c->dInst, c->cl.dev, c->cl.n(), (const T *)c->d_m,
+                                       (const T *)c->d_h, c->d_part, c->any_demag ? 1 : 0);
+           }));
+    HIPCHK(c, launch_energy_final(st, c->d_efirst, c->count, c->d_part, c->d_eout));
+    sums.resize(8 * c->count);
+    HIPCHK(c, hipMemcpyAsync(sums.data(), c->d_eout, sizeof(double) * 8 * c->count,
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return MMC_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+int mmc_create(const mmc_params *p, int precision, mmc_ctx **out)
+{
+    return create_common(p, 1, precision, out);
+}
+
+int mmc_batch_create(const mmc_params *p, int count, int precision, mmc_ctx **out)
+{
+    return create_common(p, count, precision, out);
+}
+
+void mmc_destroy(mmc_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    for (void *p : {c->d_m, c->d_h, c->d_a, c->d_khat, (void *)c->d_K, c->d_tw32, c->d_tw64,
+                    (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
+                    (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
+        if (p) hipFree(p);
+    if (!c->b_alias && c->d_b) hipFree(c->d_b);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl})
+        if (l->dev) hipFree(l->dev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *mmc_last_error(const mmc_ctx *c)
+{
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+int mmc_count(const mmc_ctx *c) { return c ? c->count : 0; }
+
+int mmc_dims(const mmc_ctx *c, int inst, int out[6])
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    out[0] = in.nx; out[1] = in.ny; out[2] = in.nz;
+    out[3] = in.Px; out[4] = in.Py; out[5] = in.Pz;
+    return MMC_OK;
+}
+
+int mmc_build_tensor(mmc_ctx *c, const char *cache_dir)
+{
+    if (!c) return MMC_EINVAL;
+    if (!c->any_demag) { c->tensor_ready = true; return MMC_OK; }
+    hipStream_t st = c->stream;
+    std::vector<char> loaded(c->count, 0);
+    if (cache_dir && *cache_dir) {
+        for (int i = 0; i < c->count; ++i) {
+            const Inst &in = c->inst[i];
+            if (!in.cal_demag) continue;
+            std::vector<double> K;
+            if (load_cache(std::string(cache_dir) + "/" + cache_name(in, c->params[i]), in, c->params[i], K)) {
+                HIPCHK(c, hipMemcpy(c->d_K + in.k_off, K.data(), sizeof(double) * K.size(),
+                                    hipMemcpyHostToDevice));
+                loaded[i] = 1;
+            }
+        }
+    }
+    // image slices: bound each launch to ~2e9 Newell evaluations
+    int max_img = 0;
+    for (auto &in : c->inst) if (in.cal_demag) max_img = std::max(max_img, in.img[0]);
+    double evals = 0;
+    for (int i = 0; i < c->count; ++i)
+        if (!loaded[i] && c->inst[i].cal_demag)
+            evals += (double)c->inst[i].NP * 162.0 * (2 * c->inst[i].img[1] + 1) * (2 * c->inst[i].img[2] + 1);
+    // build a work list of the instances still to compute
+    std::vector<Work> todo;
+    for (auto &w : c->tl.host) if (!loaded[w.inst]) todo.push_back(w);
+    if (!todo.empty()) {
+        Work *dW = nullptr;
+        HIPCHK(c, hipMalloc(&dW, sizeof(Work) * todo.size()));
+        HIPCHK(c, hipMemcpy(dW, todo.data(), sizeof(Work) * todo.size(), hipMemcpyHostToDevice));
+        const int per = std::max(1, (int)(2e9 / std::max(1.0, evals)));
+        int rc = MMC_OK;
+        for (int lo = -max_img; lo <= max_img && rc == MMC_OK; lo += per) {
+            const int hi = std::min(max_img, lo + per - 1);
+            hipError_t e = timed(c, K_TENSOR, [&] {
+                return launch_tensor(st, c->dInst, dW, (int)todo.size(), c->d_K, lo, hi,
+                                     lo == -max_img, hi == max_img);
+            });
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = fail(c, MMC_EHIP, "tensor build failed: %s", hipGetErrorString(e));
+        }
+        hipFree(dW);
+        if (rc) return rc;
+    }
+    if (cache_dir && *cache_dir) {
+        struct stat sb;
+        if (stat(cache_dir, &sb) == 0 && S_ISDIR(sb.st_mode))
+            for (int i = 0; i < c->count; ++i) {
+                const Inst &in = c->inst[i];
+                if (!in.cal_demag || loaded[i]) continue;
+                std::vector<double> K(6 * in.NP);
+                HIPCHK(c, hipMemcpy(K.data(), c->d_K + in.k_off, sizeof(double) * K.size(),
+                                    hipMemcpyDeviceToHost));
+                save_cache(std::string(cache_dir) + "/" + cache_name(in, c->params[i]), in, c->params[i], K);
+            }
+    }
+    return finish_tensor(c);
+}
+
+int mmc_set_tensor(mmc_ctx *c, int inst, const double *K)
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    if (!in.cal_demag) return fail(c, MMC_ESTATE, "instance %d has cal_demag off", inst);
+    HIPCHK(c, hipMemcpy(c->d_K + in.k_off, K, sizeof(double) * 6 * in.NP, hipMemcpyHostToDevice));
+    return finish_tensor(c);
+}
+
+int mmc_get_tensor(const mmc_ctx *cc, int inst, double *K)
+{
+    mmc_ctx *c = const_cast<mmc_ctx *>(cc);
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    if (!in.cal_demag) return fail(c, MMC_ESTATE, "instance %d has cal_demag off", inst);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(K, c->d_K + in.k_off, sizeof(double) * 6 * in.NP, hipMemcpyDeviceToHost));
+    return MMC_OK;
+}
+
+int mmc_mirror_tensor(const mmc_ctx *cc, int inst, double K0[6])
+{
+    int rc = check_inst(cc, inst);
+    if (rc) return rc;
+    for (int k = 0; k < 6; ++k) K0[k] = cc->inst[inst].K0[k];
+    return MMC_OK;
+}
+
+int mmc_set_m(mmc_ctx *c, int inst, const double *mx, const double *my, const double *mz)
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    const double *src[3] = {mx, my, mz};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) {
+        if (c->tsize == 8) {
+            HIPCHK(c, hipMemcpy((double *)c->d_m + in.m_off + k * in.N, src[k], sizeof(double) * in.N,
+                                hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> tmp(src[k], src[k] + in.N);
+            HIPCHK(c, hipMemcpy((float *)c->d_m + in.m_off + k * in.N, tmp.data(), sizeof(float) * in.N,
+                                hipMemcpyHostToDevice));
+        }
+    }
+    return MMC_OK;
+}
+
+int mmc_get_m(const mmc_ctx *cc, int inst, double *mx, double *my, double *mz)
+{
+    mmc_ctx *c = const_cast<mmc_ctx *>(cc);
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    double *dst[3] = {mx, my, mz};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) {
+        if (c->tsize == 8) {
+            HIPCHK(c, hipMemcpy(dst[k], (double *)c->d_m + in.m_off + k * in.N, sizeof(double) * in.N,
+                                hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(in.N);
+            HIPCHK(c, hipMemcpy(tmp.data(), (float *)c->d_m + in.m_off + k * in.N, sizeof(float) * in.N,
+                                hipMemcpyDeviceToHost));
+            for (long i = 0; i < in.N; ++i) dst[k][i] = tmp[i];
+        }
+    }
+    return MMC_OK;
+}
+
+int mmc_init_random_m(mmc_ctx *c, int inst)
+{
+    if (!c) return MMC_EINVAL;
+    if (inst >= c->count || inst < -1) return fail(c, MMC_EINVAL, "instance %d out of range", inst);
+    const int b0 = inst < 0 ? 0 : c->cl.first[inst];
+    const int b1 = inst < 0 ? c->cl.n() : c->cl.first[inst + 1];
+    hipError_t e = c->tsize == 4
+                       ? launch_init<float>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0, (float *)c->d_m)
+                       : launch_init<double>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0, (double *)c->d_m);
+    HIPCHK(c, e);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MMC_OK;
+}
+
+int mmc_field(mmc_ctx *c, int inst, int terms, double *hx, double *hy, double *hz)
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    if ((terms & MMC_DEMAG) && in.cal_demag) {
+        rc = c->tsize == 4 ? demag_chain<float>(c) : demag_chain<double>(c);
+        if (rc) return rc;
+    }
+    const int b0 = c->cl.first[inst], b1 = c->cl.first[inst + 1];
+    hipError_t e = c->tsize == 4
+                       ? launch_field<float>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0,
+                                             (const float *)c->d_m, (const float *)c->d_h, terms, c->d_H)
+                       : launch_field<double>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0,
+                                              (const double *)c->d_m, (const double *)c->d_h, terms, c->d_H);
+    HIPCHK(c, e);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double *dst[3] = {hx, hy, hz};
+    for (int k = 0; k < 3; ++k)
+        HIPCHK(c, hipMemcpy(dst[k], c->d_H + k * in.N, sizeof(double) * in.N, hipMemcpyDeviceToHost));
+    return MMC_OK;
+}
+
+int mmc_energy(mmc_ctx *c, int inst, double *E_total, double e_terms[5], double m_avg[3])
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    std::vector<double> s;
+    rc = c->tsize == 4 ? energy_impl<float>(c, s) : energy_impl<double>(c, s);
+    if (rc) return rc;
+    const Inst &in = c->inst[inst];
+    const double *v = s.data() + 8 * inst;
+    double tot = 0.0;
+    double et[5];
+    for (int t = 0; t < 5; ++t) {
+        et[t] = v[t] * in.e_unit;
+        tot += et[t];
+    }
+    if (E_total) *E_total = tot;
+    if (e_terms) for (int t = 0; t < 5; ++t) e_terms[t] = et[t];
+    if (m_avg) for (int k = 0; k < 3; ++k) m_avg[k] = v[5 + k] / (double)in.N;
+    return MMC_OK;
+}
+
+int mmc_set_temperature(mmc_ctx *c, int inst, double T)
+{
+    if (!c) return MMC_EINVAL;
+    if (inst >= c->count || inst < -1) return fail(c, MMC_EINVAL, "instance %d out of range", inst);
+    if (!(T > 0)) return fail(c, MMC_EINVAL, "temperature must be positive");
+    for (int i = 0; i < c->count; ++i)
+        if (inst < 0 || i == inst) c->T[i] = T;
+    std::vector<double> beta(c->count);
+    for (int i = 0; i < c->count; ++i) beta[i] = c->inst[i].e_unit / (kBoltzmann * c->T[i]);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_beta, beta.data(), sizeof(double) * c->count, hipMemcpyHostToDevice));
+    return MMC_OK;
+}
+
+int mmc_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
+{
+    if (!c) return MMC_EINVAL;
+    return c->tsize == 4 ? run_sweeps<float>(c, stage, sweep0, count)
+                         : run_sweeps<double>(c, stage, sweep0, count);
+}
+
+int mmc_sweep(mmc_ctx *c, double T, uint32_t stage, uint32_t sweep)
+{
+    int rc = mmc_set_temperature(c, -1, T);
+    if (rc) return rc;
+    return mmc_sweeps(c, stage, sweep, 1);
+}
+
+int mmc_sync(mmc_ctx *c)
+{
+    if (!c) return MMC_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MMC_OK;
+}
+
+int mmc_anneal(mmc_ctx *c, const mmc_schedule *s, mmc_energy_cb ecb, mmc_snapshot_cb scb,
+               void *user, volatile int *stop, long *stages_out)
+{
+    if (!c || !s) return MMC_EINVAL;
+    if (s->sweeps_per_stage < 0) return fail(c, MMC_EINVAL, "sweeps_per_stage < 0");
+    double T = s->T_start;
+    long stage = 0;
+    const Inst &in = c->inst[0];
+    std::vector<double> mx, my, mz;
+    int rc = MMC_OK;
+    while (T > s->T_end) {                                   // src/main.cu:801
+        if (stop && *stop) break;
+        if ((rc = mmc_set_temperature(c, -1, T))) return rc;
+        long sw = 0;
+        while (sw < s->sweeps_per_stage) {
+            if (stop && *stop) break;
+            // next sweep index at which an output is due
+            long next = s->sweeps_per_stage;
+            auto due = [&](long period) {
+                if (period <= 0) return;
+                long k = ((sw + period - 1) / period) * period;
+                next = std::min(next, k);
+            };
+            due(s->energy_period);
+            due(s->snapshot_period);
+            if (next == sw) {
+                if (s->snapshot_period > 0 && sw % s->snapshot_period == 0 && scb) {
+                    mx.resize(in.N); my.resize(in.N); mz.resize(in.N);
+                    if ((rc = mmc_get_m(c, 0, mx.data(), my.data(), mz.data()))) return rc;
+                    if (scb(user, T, stage, sw, mx.data(), my.data(), mz.data())) { if (stages_out) *stages_out = stage; return MMC_OK; }
+                }
+                if (s->energy_period > 0 && sw % s->energy_period == 0 && ecb) {
+                    double E = 0;
+                    if ((rc = mmc_energy(c, 0, &E, nullptr, nullptr))) return rc;
+                    if (ecb(user, T, stage, sw, E)) { if (stages_out) *stages_out = stage; return MMC_OK; }
+                }
+                next = sw + 1;
+                long k2 = s->sweeps_per_stage;
+                auto due2 = [&](long period) {
+                    if (period <= 0) return;
+                    long k = ((sw + 1 + period - 1) / period) * period;
+                    k2 = std::min(k2, k);
+                };
+                due2(s->energy_period);
+                due2(s->snapshot_period);
+                next = k2;
+            }
+            // run up to the next output point, in chunks so *stop is polled
+            long chunk = std::min(next - sw, 1000L);
+            if ((rc = mmc_sweeps(c, (uint32_t)stage, (uint32_t)sw, (uint32_t)chunk))) return rc;
+            if ((rc = mmc_sync(c))) return rc;
+            sw += chunk;
+        }
+        ++stage;
+        if (stop && *stop) break;
+        T = s->use_exp ? T * s->T_step : T - s->T_step;     // src/main.cu:1129-1136
+    }
+    if (stages_out) *stages_out = stage;
+    return MMC_OK;
+}
+
+int mmc_set_option(mmc_ctx *c, const char *key, long value)
+{
+    if (!c || !key) return MMC_EINVAL;
+    std::string k(key);
+    if (k == "graph") {
+        c->use_graph = value != 0;
+        return MMC_OK;
+    }
+    if (k == "graph_sweeps") {
+        if (value < 1 || value > 4096) return fail(c, MMC_EINVAL, "graph_sweeps must be in [1,4096]");
+        if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+        c->graph_sweeps = (int)value;
+        return MMC_OK;
+    }
+    if (k == "timing") {
+        if (!value) harvest(c);
+        c->timing = value != 0;
+        return MMC_OK;
+    }
+    return fail(c, MMC_EINVAL, "unknown option '%s'", key);
+}
+
+int mmc_kernel_time(mmc_ctx *c, const char *kind, long *launches, double *total_ms, int reset)
+{
+    if (!c || !kind) return MMC_EINVAL;
+    harvest(c);
+    for (int k = 0; k < kNumKinds; ++k)
+        if (!strcmp(kind, kKinds[k])) {
+            if (launches) *launches = c->launches[k];
+            if (total_ms) *total_ms = c->total_ms[k];
+            if (reset) { c->launches[k] = 0; c->total_ms[k] = 0; }
+            return MMC_OK;
+        }
+    return fail(c, MMC_EINVAL, "unknown kernel kind '%s'", kind);
+}
+
+int mmc_kernel_bytes(const mmc_ctx *cc, const char *kind, double *bytes)
+{
+    mmc_ctx *c = const_cast<mmc_ctx *>(cc);
+    if (!c || !kind || !bytes) return MMC_EINVAL;
+    const double b = (double)c->tsize, cb = 2.0 * b;
+    double tot = 0;
+    std::string k(kind);
+    for (auto &in : c->inst) {
+        const double N = (double)in.N;
+        const double spec_x = 3.0 * in.Hx * in.nz * in.ny * cb;   // A
+        const double spec_z = 3.0 * in.Hx * in.Pz * in.ny * cb;   // B
+        const double khat = 6.0 * in.Hx * in.Pz * in.Py * cb;
+        if (k == "xfwd" || k == "xinv") { if (in.cal_demag) tot += 3 * N * b + spec_x; }
+        else if (k == "zfwd" || k == "zinv") { if (in.cal_demag && in.Pz > 1) tot += spec_x + spec_z; }
+        else if (k == "ymul") { if (in.cal_demag) tot += 2 * spec_z + khat; }
+        else if (k == "metro") {
+            // one colour pass: all of m read (cell + neighbours), h and the
+            // updated m of this colour's cells
+            const double K = in.ncolors;
+            tot += 3 * N * b + (in.cal_demag ? 3 * N * b / K : 0.0) + 3 * N * b / K;
+        } else if (k == "energy") tot += 3 * N * b + (in.cal_demag ? 3 * N * b : 0.0);
+        else if (k == "tensor") tot += 6.0 * in.NP * 8;
+        else return fail(c, MMC_EINVAL, "unknown kernel kind '%s'", kind);
+    }
+    *bytes = tot;
+    return MMC_OK;
+}
+
+}  // extern "C"

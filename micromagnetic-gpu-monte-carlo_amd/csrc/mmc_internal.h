@@ -1,0 +1,124 @@
+// mmc_internal.h -- device data layout shared by the kernels and the C ABI.
+//
+// One context holds one or more instances (independent anneals).  Every
+// per-instance array lives in a pooled device buffer at an offset recorded
+// in Inst; every kernel walks a device-resident work list whose items name
+// the instance they belong to, so a single launch covers all instances.
+//
+// HBM layout of one instance (b = sizeof(T), T = float or double):
+//   m, h        [3][nz][ny][nx]            T       magnetisation, demag/Ms
+//   A           [3][Hx][nz][ny]            cpx<T>  x-spectrum, y fastest
+//   B           [3][Hx][Pz][ny]            cpx<T>  xz-spectrum (A if Pz==1)
+//   Khat        [6][Hx][Pz][Py]            cpx<T>  tensor spectrum / (PxPyPz)
+//   K           [6][Pz][Py][Px]            double  real-space tensor
+// with Hx = Px/2 + 1 (half spectrum of the real x transform).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mmc {
+
+constexpr int kThreads = 256;      // threads per workgroup (4 wave64)
+constexpr int kMaxStages = 16;     // radix stages per FFT plan
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+struct FftPlan {
+    int n;
+    int nst;
+    int radix[kMaxStages];
+};
+
+struct Inst {
+    int nx, ny, nz;
+    int Px, Py, Pz, Hx;
+    int pbc[3];           // nonzero: periodic axis
+    int img[3];           // image half-range of the tensor sum per axis
+    int ncolors;          // 2 (checkerboard) or 3 (odd periodic axis)
+    int cal_demag;
+    int tyx, tyz, ncol;   // tile shapes of the bulk-precision FFT passes; each
+                          // pass ping-pongs two LDS buffers of tile size
+    int tyx64, tyz64;     // tile shapes of the FP64 tensor transform
+    long N, NP;
+    long m_off, h_off;    // T
+    long a_off, b_off;    // cpx<T>
+    long khat_off;        // cpx<T>
+    long k_off;           // double
+    long ka_off;          // cpx<double>, tensor-build scratch
+    long twx, twy, twz;   // twiddle tables (same offsets in the FP32 and FP64 pools)
+    FftPlan plx, ply, plz;
+    double d[3];          // dx, dy, dz [m]
+    double lam[3];        // 2A / (mu0 Ms^2 d_a^2)
+    double dmi[2];        // D / (mu0 Ms^2 d_a), a = x, y
+    double kap;           // Ku1 / (mu0 Ms^2)
+    double u[3];          // unit anisotropy axis
+    double b[3];          // B / (mu0 Ms)
+    double K0[6];         // self ("mirror") tensor
+    double ms;            // Ms [A/m]
+    double e_unit;        // mu0 Ms^2 V [J]
+    uint32_t key[2];      // Philox key = seed
+};
+
+struct Work {
+    int inst, a, b, c;
+};
+
+// ---- launch wrappers (mmc_kernels.hip) ------------------------------------
+// Each takes a device work list (items [0, n)), launches one workgroup per
+// item on `st`, and returns the launch status.
+hipError_t launch_tensor(hipStream_t st, const Inst *I, const Work *W, int n,
+                         double *K, int ix_lo, int ix_hi, int first, int last);
+
+template <typename T>
+hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n,
+                       int mode, const T *real, cpx<T> *spec,
+                       const cpx<T> *tw, size_t lds);
+template <typename T>
+hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n,
+                       int mode, const cpx<T> *src, cpx<T> *dst,
+                       const cpx<T> *tw, size_t lds);
+template <typename T>
+hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n,
+                       const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw,
+                       size_t lds);
+template <typename T>
+hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
+                       cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw,
+                       size_t lds);
+template <typename TO>
+hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W,
+                              int n, const cpx<double> *ka, cpx<TO> *khat,
+                              const cpx<double> *tw, size_t lds);
+template <typename T>
+hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
+                       const cpx<T> *spec, T *real, const cpx<T> *tw,
+                       size_t lds);
+template <typename T>
+hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n,
+                        T *m, const T *h, const double *beta,
+                        const uint32_t *ctl, uint32_t sweep_off, int color,
+                        int use_demag);
+template <typename T>
+hipError_t launch_energy(hipStream_t st, const Inst *I, const Work *W, int n,
+                         const T *m, const T *h, double *partials,
+                         int use_demag);
+hipError_t launch_energy_final(hipStream_t st, const int *first, int ninst,
+                               const double *partials, double *out);
+template <typename T>
+hipError_t launch_field(hipStream_t st, const Inst *I, const Work *W, int n,
+                        const T *m, const T *h, int terms, double *H);
+template <typename T>
+hipError_t launch_init(hipStream_t st, const Inst *I, const Work *W, int n,
+                       T *m);
+hipError_t launch_set_ctl(hipStream_t st, uint32_t *ctl, uint32_t stage,
+                          uint32_t sweep_base);
+// Allow `bytes` of dynamic LDS on every FFT kernel (above the 64 KB default).
+hipError_t set_lds_limit(size_t bytes);
+
+}  // namespace mmc

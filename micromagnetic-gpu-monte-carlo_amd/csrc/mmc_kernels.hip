@@ -1,0 +1,933 @@
+// mmc_kernels.hip -- gfx950 kernels of the Monte Carlo annealing hot path.
+//
+//   k_tensor        FP64 Newell tensor with periodic-image sums
+//                   (src/demag_cu.cu:119-236 arithmetic, one launch per
+//                   slice of x images instead of src/main.cu:473-491's one
+//                   launch + sync per image)
+//   k_xfwd / k_xinv real <-> half-spectrum along x, two real rows packed in
+//                   one complex row, transposed store so y is contiguous
+//   k_zfwd / k_zinv z transform (3-D grids only)
+//   k_ymul          y forward transform, symmetric 3x3 k-space product
+//                   (src/demag_cu.cu:68-116), y inverse, in one pass
+//   k_yfwd_tensor   y transform of the tensor, scaled by 1/(PxPyPz)
+//   k_metro         Metropolis update of one colour class (mcKernel2,
+//                   src/mc_sampling.h:27-41)
+//   k_energy(+final) per-cell energy terms and <m>, compensated FP64
+//                   block partials, fixed-order final sum (src/main.cu:945-947)
+//   k_field         effective field in A/m (effectiveHKernel,
+//                   src/effective_H.h:25-33)
+//   k_init          Philox uniform-on-sphere start state
+#include "fft_lds.h"
+#include "mmc_internal.h"
+
+namespace mmc {
+
+// ============================================================================
+// small helpers
+// ============================================================================
+__device__ __forceinline__ int nbr(int i, int d, int n, int periodic)
+{
+    int j = i + d;
+    if (j < 0 || j >= n) {
+        if (!periodic) return -1;
+        j = (j + n) % n;
+    }
+    return j == i ? -1 : j;
+}
+
+__device__ __forceinline__ int axis_color(int i, int n, int periodic)
+{
+    return (periodic && n >= 3 && (n & 1) && i == n - 1) ? 2 : (i & 1);
+}
+
+// Philox4x32-10 (Salmon et al. 2011).
+__device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1)
+{
+    #pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// 23 random bits -> (2k+1)/2^24, exactly representable in FP32 and FP64.
+template <typename T>
+__device__ __forceinline__ T u01(uint32_t r)
+{
+    return T(2u * (r >> 9) + 1u) * T(1.0 / 16777216.0);
+}
+
+__device__ __forceinline__ void sincospi_t(float a, float *s, float *c) { sincospif(a, s, c); }
+__device__ __forceinline__ void sincospi_t(double a, double *s, double *c) { sincospi(a, s, c); }
+__device__ __forceinline__ float exp_t(float a) { return expf(a); }
+__device__ __forceinline__ double exp_t(double a) { return exp(a); }
+__device__ __forceinline__ float sqrt_t(float a) { return sqrtf(a); }
+__device__ __forceinline__ double sqrt_t(double a) { return sqrt(a); }
+
+// ============================================================================
+// (a) demag tensor, FP64
+// ============================================================================
+// Newell f, g (src/demag_cu.cu:16-44): same expression tree, EPSILON guards.
+#define MMC_ABS(X) (((X) > 0) ? (X) : (-(X)))
+constexpr double kEps = 1e-200;
+
+__device__ __noinline__ double newell_f(double x, double y, double z)
+{
+    x = MMC_ABS(x);
+    y = MMC_ABS(y);
+    z = MMC_ABS(z);
+    return +y / 2.0 * (z * z - x * x) * asinh(y / (sqrt(x * x + z * z) + kEps))
+           + z / 2.0 * (y * y - x * x) * asinh(z / (sqrt(x * x + y * y) + kEps))
+           - x * y * z * atan(y * z / (x * sqrt(x * x + y * y + z * z) + kEps))
+           + 1.0 / 6.0 * (2 * x * x - y * y - z * z) * sqrt(x * x + y * y + z * z);
+}
+
+__device__ __noinline__ double newell_g(double x, double y, double z)
+{
+    z = MMC_ABS(z);
+    return +x * y * z * asinh(z / (sqrt(x * x + y * y) + kEps))
+           + y / 6.0 * (3.0 * z * z - y * y) * asinh(x / (sqrt(y * y + z * z) + kEps))
+           + x / 6.0 * (3.0 * z * z - x * x) * asinh(y / (sqrt(x * x + z * z) + kEps))
+           - z * z * z / 6.0 * atan(x * y / (z * sqrt(x * x + y * y + z * z) + kEps))
+           - z * y * y / 2.0 * atan(x * z / (y * sqrt(x * x + y * y + z * z) + kEps))
+           - z * x * x / 2.0 * atan(y * z / (x * sqrt(x * x + y * y + z * z) + kEps))
+           - x * y * sqrt(x * x + y * y + z * z) / 3.0;
+}
+
+__device__ __forceinline__ int tensor_offset(int i, int n, int P, int periodic, int *off)
+{
+    if (periodic) {                         // src/demag_cu.cu:149-151
+        *off = (i + (P + 1) / 2 - 1) % P - (P + 1) / 2 + 1;
+        return 1;
+    }
+    if (i < n) { *off = i; return 1; }
+    if (i > P - n) { *off = i - P; return 1; }
+    *off = 0;
+    return 0;
+}
+
+// Sum of one component over the 2^6 corner offsets in the reference's loop
+// order (ikx, iky, ikz, ilx, ily, ilz), from the 27 distinct values F[].
+// Identical additions to evaluating f/g once per term.
+__device__ __forceinline__ double corner_sum(double acc, const double *F)
+{
+    #pragma unroll
+    for (int ikx = 0; ikx < 2; ++ikx)
+    #pragma unroll
+    for (int iky = 0; iky < 2; ++iky)
+    #pragma unroll
+    for (int ikz = 0; ikz < 2; ++ikz)
+    #pragma unroll
+    for (int ilx = 0; ilx < 2; ++ilx)
+    #pragma unroll
+    for (int ily = 0; ily < 2; ++ily)
+    #pragma unroll
+    for (int ilz = 0; ilz < 2; ++ilz) {
+        const int a = ikx - ilx + 1, b = iky - ily + 1, c = ikz - ilz + 1;
+        const double t = F[a * 9 + b * 3 + c];
+        const bool sign = ((ikx + iky + ikz + ilx + ily + ilz) % 2 == 0);
+        acc += (sign ? t : -t);
+    }
+    return acc;
+}
+
+// One thread per padded cell; x images ix_lo..ix_hi (inclusive) of this
+// launch.  Partial sums are kept unnormalised in K between launches so the
+// addition order matches a single loop over all images.
+__global__ __launch_bounds__(kThreads) void k_tensor(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     double *__restrict__ Kb, int ix_lo,
+                                                     int ix_hi, int first, int last)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long id = (long)w.a + threadIdx.x;
+    if (id >= in.NP) return;
+    const int Px = in.Px, Py = in.Py, Pz = in.Pz;
+    const int x = (int)(id % Px);
+    const int y = (int)((id / Px) % Py);
+    const int z = (int)(id / ((long)Px * Py));
+    double *K = Kb + in.k_off;
+    const long NP = in.NP;
+    int ox, oy, oz;
+    const int ux = tensor_offset(x, in.nx, Px, in.pbc[0], &ox);
+    const int uy = tensor_offset(y, in.ny, Py, in.pbc[1], &oy);
+    const int uz = tensor_offset(z, in.nz, Pz, in.pbc[2], &oz);
+    if (!(ux && uy && uz)) {
+        if (first)
+            for (int c = 0; c < 6; ++c) K[c * NP + id] = 0.0;
+        return;
+    }
+    const double dx = in.d[0], dy = in.d[1], dz = in.d[2];
+    double acc[6];
+    for (int c = 0; c < 6; ++c) acc[c] = first ? 0.0 : K[c * NP + id];
+    const int py = in.img[1], pz = in.img[2];
+    const int lo = max(ix_lo, -in.img[0]), hi = min(ix_hi, in.img[0]);
+    for (int iximg = lo; iximg <= hi; ++iximg)
+        for (int iyimg = -py; iyimg <= py; ++iyimg)
+            for (int izimg = -pz; izimg <= pz; ++izimg) {
+                const int ix = ox + iximg * Px;
+                const int iy = oy + iyimg * Py;
+                const int iz = oz + izimg * Pz;
+                double F[27];
+                #pragma unroll
+                for (int comp = 0; comp < 6; ++comp) {
+                    #pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                    #pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                    #pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double X = (ix + a - 1) * dx;
+                        const double Y = (iy + b - 1) * dy;
+                        const double Z = (iz + c - 1) * dz;
+                        double v;
+                        switch (comp) {   // src/demag_cu.cu:190-207
+                        case 0: v = newell_f(X, Y, Z); break;
+                        case 1: v = newell_g(X, Y, Z); break;
+                        case 2: v = newell_g(X, Z, Y); break;
+                        case 3: v = newell_f(Y, Z, X); break;
+                        case 4: v = newell_g(Y, Z, X); break;
+                        default: v = newell_f(Z, X, Y); break;
+                        }
+                        F[a * 9 + b * 3 + c] = v;
+                    }
+                    acc[comp] = corner_sum(acc[comp], F);
+                }
+            }
+    if (last) {
+        const double tmp = -3.14159265358 * 4.0 * dx * dy * dz;   // src/demag_cu.cu:221
+        for (int c = 0; c < 6; ++c) K[c * NP + id] = acc[c] / tmp;
+    } else {
+        for (int c = 0; c < 6; ++c) K[c * NP + id] = acc[c];
+    }
+}
+
+// ============================================================================
+// (b) FFT passes
+// ============================================================================
+extern __shared__ __align__(16) unsigned char g_smem[];
+
+// mode 0: magnetisation (m -> A), extents (nx, ny, nz), 3 components
+// mode 1: tensor (K -> KA, FP64), extents (Px, Py, Pz), 6 components
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_xfwd(const Inst *__restrict__ I,
+                                                   const Work *__restrict__ W, int mode,
+                                                   const T *__restrict__ realb,
+                                                   cpx<T> *__restrict__ specb,
+                                                   const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, z = w.b, y0 = w.c;
+    const int P = in.Px, H = in.Hx;
+    int Lx, Ly, Lz, ty;
+    const T *src;
+    cpx<T> *dst;
+    if (mode == 0) {
+        Lx = in.nx; Ly = in.ny; Lz = in.nz; ty = in.tyx;
+        src = realb + in.m_off + (long)c * in.N;
+        dst = specb + in.a_off + (long)c * H * Lz * Ly;
+    } else {
+        Lx = in.Px; Ly = in.Py; Lz = in.Pz; ty = in.tyx64;
+        src = realb + in.k_off + (long)c * in.NP;
+        dst = specb + in.ka_off + (long)c * H * Lz * Ly;
+    }
+    src += (long)z * Lx * Ly;
+    cpx<T> *s1 = s + (ty >> 1) * P;
+    const int nyt = min(ty, Ly - y0);
+    const int npair = (nyt + 1) >> 1;
+    for (int e = threadIdx.x; e < npair * P; e += kThreads) {
+        const int q = e / P, x = e - q * P;
+        T re = T(0), im = T(0);
+        if (x < Lx) {
+            const int ya = y0 + 2 * q;
+            re = src[(long)ya * Lx + x];
+            if (2 * q + 1 < nyt) im = src[(long)(ya + 1) * Lx + x];
+        }
+        s[q * P + x] = {re, im};
+    }
+    __syncthreads();
+    s = fft_rows<T>(s, s1, npair, 1, P, in.plx, twb + in.twx);
+    // split the packed spectrum: Xa = (Z_k + conj Z_-k)/2, Xb = -i(Z_k - conj Z_-k)/2
+    for (int e = threadIdx.x; e < H * npair; e += kThreads) {
+        const int k = e / npair, q = e - k * npair;
+        const cpx<T> A = s[q * P + k];
+        const cpx<T> B = s[q * P + (k ? P - k : 0)];
+        cpx<T> *o = dst + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+        o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+        if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
+                                                   const Work *__restrict__ W,
+                                                   const cpx<T> *__restrict__ specb,
+                                                   T *__restrict__ realb,
+                                                   const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, z = w.b, y0 = w.c;
+    const int P = in.Px, H = in.Hx;
+    const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
+    cpx<T> *s1 = s + (in.tyx >> 1) * P;
+    const cpx<T> *src = specb + in.a_off + (long)c * H * Lz * Ly;
+    T *dst = realb + in.h_off + (long)c * in.N + (long)z * Lx * Ly;
+    const int nyt = min(in.tyx, Ly - y0);
+    const int npair = (nyt + 1) >> 1;
+    // Z = Xa + i Xb on k < H, Hermitian completion above; stored conjugated
+    // so the forward transform computes the inverse.
+    for (int e = threadIdx.x; e < H * npair; e += kThreads) {
+        const int k = e / npair, q = e - k * npair;
+        const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+        const cpx<T> a = p[0];
+        const cpx<T> b = (2 * q + 1 < nyt) ? p[1] : cpx<T>{T(0), T(0)};
+        s[q * P + k] = {a.x - b.y, -(a.y + b.x)};
+        if (k > 0 && P - k >= H) s[q * P + P - k] = {a.x + b.y, a.y - b.x};
+    }
+    __syncthreads();
+    s = fft_rows<T>(s, s1, npair, 1, P, in.plx, twb + in.twx);
+    for (int e = threadIdx.x; e < nyt * Lx; e += kThreads) {
+        const int yy = e / Lx, x = e - yy * Lx;
+        const cpx<T> v = s[(yy >> 1) * P + x];
+        dst[(long)(y0 + yy) * Lx + x] = (yy & 1) ? -v.y : v.x;
+    }
+}
+
+// z transform: item (inst, c, kx, y0).  LDS element (y, z) at s[z*nyt + y].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_zfwd(const Inst *__restrict__ I,
+                                                   const Work *__restrict__ W, int mode,
+                                                   const cpx<T> *__restrict__ srcb,
+                                                   cpx<T> *__restrict__ dstb,
+                                                   const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, kx = w.b, y0 = w.c;
+    const int H = in.Hx, Pz = in.Pz;
+    int Ly, Lz, ty;
+    const cpx<T> *src;
+    cpx<T> *dst;
+    if (mode == 0) {
+        Ly = in.ny; Lz = in.nz; ty = in.tyz;
+        src = srcb + in.a_off + ((long)c * H + kx) * Lz * Ly;
+        dst = dstb + in.b_off + ((long)c * H + kx) * Pz * Ly;
+    } else {
+        Ly = in.Py; Lz = in.Pz; ty = in.tyz64;
+        src = srcb + in.ka_off + ((long)c * H + kx) * Lz * Ly;
+        dst = dstb + in.ka_off + ((long)c * H + kx) * Pz * Ly;
+    }
+    cpx<T> *s1 = s + Pz * ty;
+    const int nyt = min(ty, Ly - y0);
+    for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
+        const int zz = e / nyt, yy = e - zz * nyt;
+        s[e] = zz < Lz ? src[(long)zz * Ly + y0 + yy] : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, twb + in.twz);
+    for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
+        const int zz = e / nyt, yy = e - zz * nyt;
+        dst[(long)zz * Ly + y0 + yy] = s[e];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_zinv(const Inst *__restrict__ I,
+                                                   const Work *__restrict__ W,
+                                                   const cpx<T> *__restrict__ srcb,
+                                                   cpx<T> *__restrict__ dstb,
+                                                   const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, kx = w.b, y0 = w.c;
+    const int H = in.Hx, Pz = in.Pz, Ly = in.ny, Lz = in.nz;
+    cpx<T> *s1 = s + Pz * in.tyz;
+    const cpx<T> *src = srcb + in.b_off + ((long)c * H + kx) * Pz * Ly;
+    cpx<T> *dst = dstb + in.a_off + ((long)c * H + kx) * Lz * Ly;
+    const int nyt = min(in.tyz, Ly - y0);
+    for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
+        const int zz = e / nyt, yy = e - zz * nyt;
+        s[e] = cconj(src[(long)zz * Ly + y0 + yy]);
+    }
+    __syncthreads();
+    s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, twb + in.twz);
+    for (int e = threadIdx.x; e < Lz * nyt; e += kThreads) {
+        const int zz = e / nyt, yy = e - zz * nyt;
+        dst[(long)zz * Ly + y0 + yy] = cconj(s[e]);
+    }
+}
+
+// y forward, symmetric 3x3 product with Khat, y inverse.  Item (inst, kx0,
+// kz, ncol): columns kx0..kx0+ncol-1.  LDS row (comp, col) = comp*ncol+col.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
+                                                   const Work *__restrict__ W,
+                                                   cpx<T> *__restrict__ bufb,
+                                                   const cpx<T> *__restrict__ khatb,
+                                                   const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int kx0 = w.a, kz = w.b, ncol = w.c;
+    cpx<T> *s1 = s + 3 * in.ncol * in.Py;
+    const int H = in.Hx, Pz = in.Pz, Py = in.Py, Ly = in.ny;
+    cpx<T> *buf = bufb + in.b_off;
+    const cpx<T> *kh = khatb + in.khat_off;
+    const long cstride = (long)H * Pz * Ly;    // component stride in B
+    const long kstride = (long)H * Pz * Py;    // component stride in Khat
+    const int rows = 3 * ncol;
+    for (int e = threadIdx.x; e < rows * Py; e += kThreads) {
+        const int r = e / Py, y = e - r * Py;
+        const int comp = r / ncol, col = r - comp * ncol;
+        cpx<T> v = {T(0), T(0)};
+        if (y < Ly) v = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+        s[e] = v;
+    }
+    __syncthreads();
+    s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, twb + in.twy);
+    s1 = (s == s1) ? reinterpret_cast<cpx<T> *>(g_smem) : s1;
+    for (int e = threadIdx.x; e < ncol * Py; e += kThreads) {
+        const int col = e / Py, ky = e - col * Py;
+        const long ki = ((long)(kx0 + col) * Pz + kz) * Py + ky;
+        const cpx<T> kxx = kh[0 * kstride + ki], kxy = kh[1 * kstride + ki];
+        const cpx<T> kxz = kh[2 * kstride + ki], kyy = kh[3 * kstride + ki];
+        const cpx<T> kyz = kh[4 * kstride + ki], kzz = kh[5 * kstride + ki];
+        cpx<T> *px = s + (0 * ncol + col) * Py + ky;
+        cpx<T> *py = s + (1 * ncol + col) * Py + ky;
+        cpx<T> *pz = s + (2 * ncol + col) * Py + ky;
+        const cpx<T> mx = *px, my = *py, mz = *pz;
+        const cpx<T> hx = cadd(cadd(cmul(mx, kxx), cmul(my, kxy)), cmul(mz, kxz));
+        const cpx<T> hy = cadd(cadd(cmul(mx, kxy), cmul(my, kyy)), cmul(mz, kyz));
+        const cpx<T> hz = cadd(cadd(cmul(mx, kxz), cmul(my, kyz)), cmul(mz, kzz));
+        *px = cconj(hx);
+        *py = cconj(hy);
+        *pz = cconj(hz);
+    }
+    __syncthreads();
+    s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, twb + in.twy);
+    for (int e = threadIdx.x; e < rows * Ly; e += kThreads) {
+        const int r = e / Ly, y = e - r * Ly;
+        const int comp = r / ncol, col = r - comp * ncol;
+        buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y] = cconj(s[r * Py + y]);
+    }
+}
+
+// Tensor y transform (FP64), item (inst, comp, kx, kz); writes Khat / NP.
+template <typename TO>
+__global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict__ I,
+                                                          const Work *__restrict__ W,
+                                                          const cpx<double> *__restrict__ kab,
+                                                          cpx<TO> *__restrict__ khatb,
+                                                          const cpx<double> *__restrict__ twb)
+{
+    cpx<double> *s = reinterpret_cast<cpx<double> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int comp = w.a, kx = w.b, kz = w.c;
+    const int H = in.Hx, Pz = in.Pz, Py = in.Py;
+    const long off = ((long)comp * H + kx) * Pz * Py + (long)kz * Py;
+    const cpx<double> *src = kab + in.ka_off + off;
+    cpx<TO> *dst = khatb + in.khat_off + off;
+    for (int e = threadIdx.x; e < Py; e += kThreads) s[e] = src[e];
+    __syncthreads();
+    s = fft_rows<double>(s, s + Py, 1, 1, Py, in.ply, twb + in.twy);
+    const double scale = 1.0 / (double)in.NP;
+    for (int e = threadIdx.x; e < Py; e += kThreads)
+        dst[e] = {TO(s[e].x * scale), TO(s[e].y * scale)};
+}
+
+// ============================================================================
+// (c)+(d) local field stencil and Metropolis update
+// ============================================================================
+template <typename T> struct V3 { T x, y, z; };
+
+template <typename T>
+__device__ __forceinline__ V3<T> load_m(const T *m, long N, long i)
+{
+    return {m[i], m[N + i], m[2 * N + i]};
+}
+
+// Neighbour sums of one cell (reduced units, field per Ms):
+//   ex  = sum_a lam_a * sum_{j in nbrs_a} m_j      (exchange, neighbours only)
+//   dmi = interfacial DMI bond field
+//   cnt = sum_a lam_a * (#neighbours along a)       (Laplacian diagonal)
+template <typename T>
+__device__ __forceinline__ void stencil(const Inst &in, const T *m, int x, int y, int z,
+                                        V3<T> &ex, V3<T> &dmi, T &cnt)
+{
+    const long N = in.N;
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    ex = {T(0), T(0), T(0)};
+    dmi = {T(0), T(0), T(0)};
+    cnt = T(0);
+    const long row = (long)nx * (y + (long)ny * z);
+    {   // x bonds
+        const T lam = T(in.lam[0]), dm = T(in.dmi[0]);
+        const int jp = nbr(x, +1, nx, in.pbc[0]), jm = nbr(x, -1, nx, in.pbc[0]);
+        V3<T> p = {T(0), T(0), T(0)}, q = {T(0), T(0), T(0)};
+        if (jp >= 0) { p = load_m(m, N, row + jp); cnt += lam; }
+        if (jm >= 0) { q = load_m(m, N, row + jm); cnt += lam; }
+        ex.x += lam * (p.x + q.x); ex.y += lam * (p.y + q.y); ex.z += lam * (p.z + q.z);
+        dmi.x += dm * (p.z - q.z);
+        dmi.z -= dm * (p.x - q.x);
+    }
+    {   // y bonds
+        const T lam = T(in.lam[1]), dm = T(in.dmi[1]);
+        const int jp = nbr(y, +1, ny, in.pbc[1]), jm = nbr(y, -1, ny, in.pbc[1]);
+        V3<T> p = {T(0), T(0), T(0)}, q = {T(0), T(0), T(0)};
+        if (jp >= 0) { p = load_m(m, N, x + (long)nx * (jp + (long)ny * z)); cnt += lam; }
+        if (jm >= 0) { q = load_m(m, N, x + (long)nx * (jm + (long)ny * z)); cnt += lam; }
+        ex.x += lam * (p.x + q.x); ex.y += lam * (p.y + q.y); ex.z += lam * (p.z + q.z);
+        dmi.y += dm * (p.z - q.z);
+        dmi.z -= dm * (p.y - q.y);
+    }
+    if (nz > 1) {   // z bonds: exchange only
+        const T lam = T(in.lam[2]);
+        const int jp = nbr(z, +1, nz, in.pbc[2]), jm = nbr(z, -1, nz, in.pbc[2]);
+        V3<T> p = {T(0), T(0), T(0)}, q = {T(0), T(0), T(0)};
+        if (jp >= 0) { p = load_m(m, N, x + (long)nx * (y + (long)ny * jp)); cnt += lam; }
+        if (jm >= 0) { q = load_m(m, N, x + (long)nx * (y + (long)ny * jm)); cnt += lam; }
+        ex.x += lam * (p.x + q.x); ex.y += lam * (p.y + q.y); ex.z += lam * (p.z + q.z);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T quad_k0(const Inst &in, V3<T> a)
+{
+    const T k0 = T(in.K0[0]), k1 = T(in.K0[1]), k2 = T(in.K0[2]);
+    const T k3 = T(in.K0[3]), k4 = T(in.K0[4]), k5 = T(in.K0[5]);
+    return k0 * a.x * a.x + k3 * a.y * a.y + k5 * a.z * a.z +
+           T(2) * (k1 * a.x * a.y + k2 * a.x * a.z + k4 * a.y * a.z);
+}
+
+// One colour class.  dE (units mu0 Ms^2 V) =
+//   -d.(ex + dmi + b + h - K0 m) - kap[(m'.u)^2 - (m.u)^2] - [m'K0m' - mK0m]/2
+// accept if dE <= 0 or u < exp(-dE * beta), beta = mu0 Ms^2 V / (kB T).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_metro(const Inst *__restrict__ I,
+                                                    const Work *__restrict__ W,
+                                                    T *__restrict__ mb,
+                                                    const T *__restrict__ hb,
+                                                    const double *__restrict__ beta,
+                                                    const uint32_t *__restrict__ ctl,
+                                                    uint32_t sweep_off, int color,
+                                                    int use_demag)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long i = (long)w.a + threadIdx.x;
+    if (i >= in.N) return;
+    const int nx = in.nx, ny = in.ny;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const int col = (axis_color(x, nx, in.pbc[0]) + axis_color(y, ny, in.pbc[1]) +
+                     axis_color(z, in.nz, in.pbc[2])) % in.ncolors;
+    if (col != color) return;
+    T *m = mb + in.m_off;
+    const long N = in.N;
+    const V3<T> mi = load_m(m, N, i);
+    V3<T> ex, dmi;
+    T cnt;
+    stencil(in, m, x, y, z, ex, dmi, cnt);
+    V3<T> g = {ex.x + dmi.x + T(in.b[0]), ex.y + dmi.y + T(in.b[1]), ex.z + dmi.z + T(in.b[2])};
+    if (use_demag) {
+        const T *h = hb + in.h_off;
+        const T k0 = T(in.K0[0]), k1 = T(in.K0[1]), k2 = T(in.K0[2]);
+        const T k3 = T(in.K0[3]), k4 = T(in.K0[4]), k5 = T(in.K0[5]);
+        g.x += h[i] - (k0 * mi.x + k1 * mi.y + k2 * mi.z);
+        g.y += h[N + i] - (k1 * mi.x + k3 * mi.y + k4 * mi.z);
+        g.z += h[2 * N + i] - (k2 * mi.x + k4 * mi.y + k5 * mi.z);
+    }
+    const uint32_t sweep = ctl[1] + sweep_off;
+    const uint4 rr = philox10(make_uint4((uint32_t)i, sweep, ctl[0], 0u), in.key[0], in.key[1]);
+    const T zt = T(1) - T(2) * u01<T>(rr.x);
+    const T st = sqrt_t(max(T(0), T(1) - zt * zt));
+    T sn, cs;
+    sincospi_t(T(2) * u01<T>(rr.y), &sn, &cs);
+    const V3<T> mn = {st * cs, st * sn, zt};
+    const V3<T> d = {mn.x - mi.x, mn.y - mi.y, mn.z - mi.z};
+    T dE = -(d.x * g.x + d.y * g.y + d.z * g.z);
+    const T ux = T(in.u[0]), uy = T(in.u[1]), uz = T(in.u[2]);
+    const T pu = mn.x * ux + mn.y * uy + mn.z * uz;
+    const T ou = mi.x * ux + mi.y * uy + mi.z * uz;
+    dE -= T(in.kap) * (pu * pu - ou * ou);
+    if (use_demag) dE -= T(0.5) * (quad_k0(in, mn) - quad_k0(in, mi));
+    const T bt = T(beta[w.inst]);
+    if (dE <= T(0) || u01<T>(rr.z) < exp_t(-dE * bt)) {
+        m[i] = mn.x;
+        m[N + i] = mn.y;
+        m[2 * N + i] = mn.z;
+    }
+}
+
+// ============================================================================
+// (e) energy terms and <m>, compensated FP64 reductions
+// ============================================================================
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    const double s = a.hi + b.hi;
+    const double v = s - a.hi;
+    double e = (a.hi - (s - v)) + (b.hi - v);
+    e += a.lo + b.lo;
+    const double hi = s + e;
+    return {hi, e - (hi - s)};
+}
+
+__device__ __forceinline__ DD dd_shfl_down(DD a, int off)
+{
+    return {__shfl_down(a.hi, off, 64), __shfl_down(a.lo, off, 64)};
+}
+
+// Fixed-order block reduction of NQ double-double quantities; result valid
+// in thread 0.
+template <int NQ>
+__device__ __forceinline__ void block_reduce(DD *v)
+{
+    __shared__ DD red[kThreads / 64][NQ];
+    #pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        for (int off = 32; off > 0; off >>= 1) v[q] = dd_add(v[q], dd_shfl_down(v[q], off));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < NQ; ++q) red[wid][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NQ; ++q) {
+            DD a = red[0][q];
+            for (int k = 1; k < kThreads / 64; ++k) a = dd_add(a, red[k][q]);
+            v[q] = a;
+        }
+}
+
+// Per-cell energy densities (units mu0 Ms^2):
+//   e_ex = -m.(ex - cnt m)/2, e_dmi = -m.dmi/2, e_anis = -kap (m.u)^2,
+//   e_zee = -m.b, e_demag = -m.h/2;  plus m itself.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_energy(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     const T *__restrict__ mb,
+                                                     const T *__restrict__ hb,
+                                                     double *__restrict__ part,
+                                                     int use_demag)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long i = (long)w.a + threadIdx.x;
+    DD v[8];
+    #pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = {0.0, 0.0};
+    if (i < in.N) {
+        const int nx = in.nx, ny = in.ny;
+        const int x = (int)(i % nx);
+        const long r = i / nx;
+        const int y = (int)(r % ny);
+        const int z = (int)(r / ny);
+        const T *m = mb + in.m_off;
+        const long N = in.N;
+        const V3<T> mi = load_m(m, N, i);
+        V3<T> ex, dmi;
+        T cnt;
+        stencil(in, m, x, y, z, ex, dmi, cnt);
+        const T hex_x = ex.x - cnt * mi.x, hex_y = ex.y - cnt * mi.y, hex_z = ex.z - cnt * mi.z;
+        const T mu = mi.x * T(in.u[0]) + mi.y * T(in.u[1]) + mi.z * T(in.u[2]);
+        v[0].hi = double(T(-0.5) * (mi.x * hex_x + mi.y * hex_y + mi.z * hex_z));
+        v[1].hi = double(T(-0.5) * (mi.x * dmi.x + mi.y * dmi.y + mi.z * dmi.z));
+        v[2].hi = double(-T(in.kap) * mu * mu);
+        v[3].hi = double(-(mi.x * T(in.b[0]) + mi.y * T(in.b[1]) + mi.z * T(in.b[2])));
+        if (use_demag) {
+            const T *h = hb + in.h_off;
+            v[4].hi = double(T(-0.5) * (mi.x * h[i] + mi.y * h[N + i] + mi.z * h[2 * N + i]));
+        }
+        v[5].hi = double(mi.x);
+        v[6].hi = double(mi.y);
+        v[7].hi = double(mi.z);
+    }
+    block_reduce<8>(v);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 8; ++q) {
+            part[(long)blockIdx.x * 16 + 2 * q] = v[q].hi;
+            part[(long)blockIdx.x * 16 + 2 * q + 1] = v[q].lo;
+        }
+}
+
+// One workgroup per instance; items first[inst] .. first[inst+1]-1.
+__global__ __launch_bounds__(kThreads) void k_energy_final(const int *__restrict__ first,
+                                                           const double *__restrict__ part,
+                                                           double *__restrict__ out)
+{
+    const int inst = blockIdx.x;
+    const int b0 = first[inst], b1 = first[inst + 1];
+    DD v[8];
+    #pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = {0.0, 0.0};
+    for (int b = b0 + threadIdx.x; b < b1; b += kThreads)
+        #pragma unroll
+        for (int q = 0; q < 8; ++q)
+            v[q] = dd_add(v[q], DD{part[(long)b * 16 + 2 * q], part[(long)b * 16 + 2 * q + 1]});
+    block_reduce<8>(v);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 8; ++q) out[inst * 8 + q] = v[q].hi + v[q].lo;
+}
+
+// Effective field in A/m of the selected terms (bits: 1 exch, 2 dmi,
+// 4 anis, 8 zeeman, 16 demag).  Output [3][N] doubles of the item's instance.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_field(const Inst *__restrict__ I,
+                                                    const Work *__restrict__ W,
+                                                    const T *__restrict__ mb,
+                                                    const T *__restrict__ hb, int terms,
+                                                    double *__restrict__ H)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long i = (long)w.a + threadIdx.x;
+    if (i >= in.N) return;
+    const int nx = in.nx, ny = in.ny;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const T *m = mb + in.m_off;
+    const long N = in.N;
+    const V3<T> mi = load_m(m, N, i);
+    V3<T> ex, dmi;
+    T cnt;
+    stencil(in, m, x, y, z, ex, dmi, cnt);
+    T hx = T(0), hy = T(0), hz = T(0);
+    if (terms & 1) { hx += ex.x - cnt * mi.x; hy += ex.y - cnt * mi.y; hz += ex.z - cnt * mi.z; }
+    if (terms & 2) { hx += dmi.x; hy += dmi.y; hz += dmi.z; }
+    if (terms & 4) {
+        const T mu = mi.x * T(in.u[0]) + mi.y * T(in.u[1]) + mi.z * T(in.u[2]);
+        const T f = T(2) * T(in.kap) * mu;
+        hx += f * T(in.u[0]); hy += f * T(in.u[1]); hz += f * T(in.u[2]);
+    }
+    if (terms & 8) { hx += T(in.b[0]); hy += T(in.b[1]); hz += T(in.b[2]); }
+    if ((terms & 16) && in.cal_demag) {
+        const T *h = hb + in.h_off;
+        hx += h[i]; hy += h[N + i]; hz += h[2 * N + i];
+    }
+    H[i] = in.ms * double(hx);
+    H[N + i] = in.ms * double(hy);
+    H[2 * N + i] = in.ms * double(hz);
+}
+
+// Uniform-on-sphere start state, Philox counter (cell, 0, 0, 1).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_init(const Inst *__restrict__ I,
+                                                   const Work *__restrict__ W,
+                                                   T *__restrict__ mb)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long i = (long)w.a + threadIdx.x;
+    if (i >= in.N) return;
+    const uint4 rr = philox10(make_uint4((uint32_t)i, 0u, 0u, 1u), in.key[0], in.key[1]);
+    const T zt = T(1) - T(2) * u01<T>(rr.x);
+    const T st = sqrt_t(max(T(0), T(1) - zt * zt));
+    T sn, cs;
+    sincospi_t(T(2) * u01<T>(rr.y), &sn, &cs);
+    T *m = mb + in.m_off;
+    m[i] = st * cs;
+    m[in.N + i] = st * sn;
+    m[2 * in.N + i] = zt;
+}
+
+__global__ void k_set_ctl(uint32_t *ctl, uint32_t stage, uint32_t sweep_base)
+{
+    if (threadIdx.x == 0) {
+        ctl[0] = stage;
+        ctl[1] = sweep_base;
+    }
+}
+
+// ============================================================================
+// launch wrappers
+// ============================================================================
+static inline hipError_t post() { return hipGetLastError(); }
+
+hipError_t launch_tensor(hipStream_t st, const Inst *I, const Work *W, int n, double *K,
+                         int ix_lo, int ix_hi, int first, int last)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tensor, dim3(n), dim3(kThreads), 0, st, I, W, K, ix_lo, ix_hi, first,
+                       last);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode,
+                       const T *real, cpx<T> *spec, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, real, spec, tw);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode,
+                       const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, src, dst, tw);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, const cpx<T> *src,
+                       cpx<T> *dst, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, src, dst, tw);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, cpx<T> *buf,
+                       const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ymul<T>, dim3(n), dim3(kThreads), lds, st, I, W, buf, khat, tw);
+    return post();
+}
+
+template <typename TO>
+hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W, int n,
+                              const cpx<double> *ka, cpx<TO> *khat, const cpx<double> *tw,
+                              size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_yfwd_tensor<TO>, dim3(n), dim3(kThreads), lds, st, I, W, ka, khat, tw);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
+                       const cpx<T> *spec, T *real, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, spec, real, tw);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n, T *m, const T *h,
+                        const double *beta, const uint32_t *ctl, uint32_t sweep_off, int color,
+                        int use_demag)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_metro<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, beta, ctl,
+                       sweep_off, color, use_demag);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_energy(hipStream_t st, const Inst *I, const Work *W, int n, const T *m,
+                         const T *h, double *partials, int use_demag)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, partials,
+                       use_demag);
+    return post();
+}
+
+hipError_t launch_energy_final(hipStream_t st, const int *first, int ninst,
+                               const double *partials, double *out)
+{
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy_final, dim3(ninst), dim3(kThreads), 0, st, first, partials, out);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_field(hipStream_t st, const Inst *I, const Work *W, int n, const T *m,
+                        const T *h, int terms, double *H)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, terms, H);
+    return post();
+}
+
+template <typename T>
+hipError_t launch_init(hipStream_t st, const Inst *I, const Work *W, int n, T *m)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init<T>, dim3(n), dim3(kThreads), 0, st, I, W, m);
+    return post();
+}
+
+hipError_t launch_set_ctl(hipStream_t st, uint32_t *ctl, uint32_t stage, uint32_t sweep_base)
+{
+    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, ctl, stage, sweep_base);
+    return post();
+}
+
+template <typename T>
+static hipError_t set_lds_t(int bytes)
+{
+    hipError_t e = hipSuccess;
+    auto one = [&](const void *f) {
+        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (r != hipSuccess) e = r;
+    };
+    one((const void *)k_xfwd<T>);
+    one((const void *)k_xinv<T>);
+    one((const void *)k_zfwd<T>);
+    one((const void *)k_zinv<T>);
+    one((const void *)k_ymul<T>);
+    one((const void *)k_yfwd_tensor<T>);
+    return e;
+}
+
+hipError_t set_lds_limit(size_t bytes)
+{
+    hipError_t e = set_lds_t<float>((int)bytes);
+    if (e != hipSuccess) return e;
+    return set_lds_t<double>((int)bytes);
+}
+
+#define MMC_INST(T)                                                                           \
+    template hipError_t launch_xfwd<T>(hipStream_t, const Inst *, const Work *, int, int,     \
+                                       const T *, cpx<T> *, const cpx<T> *, size_t);          \
+    template hipError_t launch_zfwd<T>(hipStream_t, const Inst *, const Work *, int, int,     \
+                                       const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
+    template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int,          \
+                                       const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
+    template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, cpx<T> *, \
+                                       const cpx<T> *, const cpx<T> *, size_t);               \
+    template hipError_t launch_yfwd_tensor<T>(hipStream_t, const Inst *, const Work *, int,   \
+                                              const cpx<double> *, cpx<T> *,                  \
+                                              const cpx<double> *, size_t);                   \
+    template hipError_t launch_xinv<T>(hipStream_t, const Inst *, const Work *, int,          \
+                                       const cpx<T> *, T *, const cpx<T> *, size_t);          \
+    template hipError_t launch_metro<T>(hipStream_t, const Inst *, const Work *, int, T *,    \
+                                        const T *, const double *, const uint32_t *,          \
+                                        uint32_t, int, int);                                  \
+    template hipError_t launch_energy<T>(hipStream_t, const Inst *, const Work *, int,        \
+                                         const T *, const T *, double *, int);                \
+    template hipError_t launch_field<T>(hipStream_t, const Inst *, const Work *, int,         \
+                                        const T *, const T *, int, double *);                 \
+    template hipError_t launch_init<T>(hipStream_t, const Inst *, const Work *, int, T *);
+
+MMC_INST(float)
+MMC_INST(double)
+
+}  // namespace mmc

@@ -1,0 +1,47 @@
+"""Shared builders for the parity tests (configs from the anneal workloads)."""
+import numpy as np
+
+# material of the reference README example / 64x64 and 256x256 anneals
+FILM = dict(dx=1e-9, dy=1e-9, dz=0.8e-9, ms=1.6e6, aex=16e-12, dind=6e-3,
+            anis_ux=0.0, anis_uy=0.0, anis_uz=1.0, ku1=2e6, bext_x=0.0,
+            bext_y=0.0, bext_z=0.3, cal_demag=1, seed=256)
+# 3-D film material (128x128x16 workload)
+FILM3D = dict(dx=1e-9, dy=1e-9, dz=1e-9, ms=1.6e6, aex=13e-12, dind=3e-3,
+              anis_ux=0.0, anis_uy=0.0, anis_uz=1.0, ku1=8e5, bext_x=0.0,
+              bext_y=0.0, bext_z=0.2, cal_demag=1, seed=7)
+
+
+def film(**kw):
+    d = dict(FILM)
+    d.update(kw)
+    return d
+
+
+def random_m(n, seed):
+    """Unit vectors, shape (3, nz, ny, nx), from a seeded normal draw."""
+    nx, ny, nz = n
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal((3, nz, ny, nx))
+    return m / np.linalg.norm(m, axis=0)
+
+
+def rel_inf(a, b):
+    """max|a-b| / max|b| (norm-wise relative error)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+
+def pair(orc, mmc, precision=0, shared_tensor=True, **kw):
+    """Oracle and library instance on the same parameters (tensor shared)."""
+    o = orc.Oracle(**kw)
+    p = mmc.make_params(**kw)
+    c = mmc.Context(p, precision)
+    if kw.get("cal_demag", 1):
+        o.build_tensor()
+        if shared_tensor:
+            c.set_tensor(o.tensor())
+        else:
+            c.build_tensor()
+    return o, c

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP library against the FP64 oracle on identical inputs.
+
+FP64 mode: fields and energies within 1e-12 (norm-wise relative), using the
+oracle's tensor uploaded to the device so the comparison isolates the
+convolution, stencil and reductions from the Newell cancellation noise.
+FP32-bulk mode: within the bounds written in each test.
+"""
+import numpy as np
+import pytest
+
+from helpers import film, FILM3D, pair, random_m, rel_inf
+
+pytestmark = pytest.mark.gpu
+
+TERMS = [("exch", 1), ("dmi", 2), ("anis", 4), ("zeeman", 8), ("demag", 16), ("all", 31)]
+
+CASES = [
+    ("film64_open", dict(nx=64, ny=64, nz=1)),
+    ("film48x40_pbc", dict(nx=48, ny=40, nz=1, pbc_x=2, pbc_y=3)),
+    ("odd_pbc", dict(nx=15, ny=12, nz=1, pbc_x=1)),
+    ("box3d_open", dict(nx=12, ny=10, nz=4)),
+    ("box3d_pbcz", dict(nx=10, ny=8, nz=3, pbc_z=2)),
+    ("ragged_3d", dict(nx=7, ny=9, nz=5, pbc_y=1)),
+]
+
+
+@pytest.mark.parametrize("name,geo", CASES, ids=[c[0] for c in CASES])
+def test_field_fp64(orc, mmc, name, geo):
+    kw = film(**geo) if "3d" not in name else dict(FILM3D, **geo)
+    o, c = pair(orc, mmc, **kw)
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 11)
+    o.set_m(m)
+    c.set_m(m)
+    for tname, bits in TERMS:
+        Ho = o.field(bits)
+        Hg = c.field(bits)
+        err = rel_inf(Hg, Ho)
+        assert err <= 1e-12, f"{name} {tname}: rel err {err:.3e}"
+
+
+@pytest.mark.parametrize("name,geo", CASES, ids=[c[0] for c in CASES])
+def test_energy_fp64(orc, mmc, name, geo):
+    kw = film(**geo) if "3d" not in name else dict(FILM3D, **geo)
+    o, c = pair(orc, mmc, **kw)
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 12)
+    o.set_m(m)
+    c.set_m(m)
+    Eo = o.energy()
+    Eg = c.energy()
+    scale = sum(abs(Eo[k]) for k in ("exch", "dmi", "anis", "zeeman", "demag"))
+    for k in ("total", "exch", "dmi", "anis", "zeeman", "demag"):
+        assert abs(Eg[k] - Eo[k]) <= 1e-12 * scale, f"{name} {k}: {Eg[k]} vs {Eo[k]}"
+    assert np.allclose(Eg["m_avg"], Eo["m_avg"], rtol=0, atol=1e-14)
+
+
+def test_demag_impulse_response(orc, mmc):
+    """One-hot m (cell 0 along x, the reference's USE_ONE_HOT_MX_INIT,
+    src/main.cu:302-310): the demag field is Ms times one tensor column."""
+    kw = film(nx=20, ny=12, nz=1)
+    o, c = pair(orc, mmc, **kw)
+    m = np.zeros((3, 1, 12, 20))
+    m[0, 0, 0, 0] = 1.0
+    c.set_m(m)
+    H = c.field(16) / kw["ms"]
+    K = o.tensor()
+    # offsets (x, y) >= 0 sit at padded index (x, y)
+    for comp, kidx in ((0, 0), (1, 1), (2, 2)):
+        ref = K[kidx, 0, :12, :20]
+        assert rel_inf(H[comp, 0], ref) <= 1e-12
+
+
+def test_tensor_gpu_vs_oracle(orc, mmc):
+    """GPU Newell tensor against the oracle's (same formula, same addition
+    order; device vs host libm differ by ulps, amplified by cancellation)."""
+    for geo in (dict(nx=8, ny=8, nz=1), dict(nx=6, ny=5, nz=3, pbc_x=2, pbc_z=1),
+                dict(nx=16, ny=16, nz=1, pbc_x=3, pbc_y=3)):
+        kw = film(**geo)
+        o = orc.Oracle(**kw)
+        o.build_tensor()
+        c = mmc.Context(mmc.make_params(**kw))
+        c.build_tensor()
+        Kg = c.tensor()
+        Ko = o.tensor()
+        assert rel_inf(Kg, Ko) <= 1e-9, geo
+        assert np.allclose(c.mirror(), o.mirror(), rtol=0, atol=1e-9)
+
+
+def test_init_random_matches_oracle(orc, mmc):
+    kw = film(nx=33, ny=17, nz=2, seed=99)
+    o = orc.Oracle(**kw)
+    o.init_random_m(99)
+    c = mmc.Context(mmc.make_params(**kw))
+    c.init_random_m()
+    assert rel_inf(c.m(), o.m()) <= 1e-15
+
+
+@pytest.mark.parametrize("geo", [dict(nx=16, ny=16, nz=1),
+                                 dict(nx=12, ny=10, nz=1, pbc_x=1, pbc_y=1),
+                                 dict(nx=9, ny=8, nz=1, pbc_x=2),
+                                 dict(nx=8, ny=6, nz=3, pbc_z=1)],
+                         ids=["open", "pbc", "odd_pbc_3colors", "3d"])
+def test_sweep_trajectory_fp64(orc, mmc, geo):
+    """Same Philox stream, same colour order, frozen demag: identical
+    accept/reject decisions, so m agrees to rounding after several sweeps."""
+    kw = film(**geo)
+    o, c = pair(orc, mmc, **kw)
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 5)
+    o.set_m(m)
+    c.set_m(m)
+    for s, T in enumerate((900.0, 300.0, 60.0)):
+        o.sweep(T, 2, s)
+        c.sweep(T, 2, s)
+    assert rel_inf(c.m(), o.m()) <= 1e-10
+
+
+def test_fp32_bulk_bounds(orc, mmc):
+    """FP32 FFT/stencil against the FP64 oracle (FP64 tensor, downcast K^)."""
+    kw = film(nx=64, ny=64, nz=1)
+    o, c = pair(orc, mmc, precision=1, **kw)
+    m = random_m((64, 64, 1), 3)
+    o.set_m(m)
+    c.set_m(m)
+    assert rel_inf(c.field(31), o.field(31)) <= 2e-6
+    assert rel_inf(c.field(16), o.field(16)) <= 2e-6
+    Eo, Eg = o.energy(), c.energy()
+    scale = sum(abs(Eo[k]) for k in ("exch", "dmi", "anis", "zeeman", "demag"))
+    assert abs(Eg["total"] - Eo["total"]) <= 1e-6 * scale
+
+
+def test_determinism_and_batch(orc, mmc):
+    """Bit-identical reruns; a batch of uneven instances gives each
+    instance exactly what its own single context gives."""
+    geos = [film(nx=16, ny=16, nz=1, seed=1), film(nx=24, ny=20, nz=1, pbc_x=1, seed=2),
+            film(nx=12, ny=12, nz=2, bext_z=0.5, seed=3)]
+    singles = []
+    for kw in geos:
+        c = mmc.Context(mmc.make_params(**kw), 1)
+        c.build_tensor()
+        c.init_random_m()
+        c.set_temperature(500.0)
+        c.sweeps(0, 0, 5)
+        singles.append(c.m())
+    b = mmc.Context([mmc.make_params(**kw) for kw in geos], 1)
+    b.build_tensor()
+    b.init_random_m()
+    b.set_temperature(500.0)
+    b.sweeps(0, 0, 5)
+    for i, ref in enumerate(singles):
+        assert np.array_equal(b.m(i), ref), f"instance {i}"
+    b2 = mmc.Context([mmc.make_params(**kw) for kw in geos], 1)
+    b2.build_tensor()
+    b2.init_random_m()
+    b2.set_temperature(500.0)
+    b2.sweeps(0, 0, 5)
+    for i in range(3):
+        assert np.array_equal(b2.m(i), b.m(i))
