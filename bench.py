@@ -1,0 +1,192 @@
+"""Benchmark: Metropolis spin-update throughput of the annealing hot path.
+
+Workload (one step = one sweep = one Metropolis attempt per cell, preceded by
+the demag convolution of the pre-sweep state): the 256x256x1 film,
+dx=dy=1 nm, dz=0.8 nm, Ms=1.6e6 A/m, Aex=16 pJ/m, Dind=6 mJ/m^2,
+Ku1=2 MJ/m^3 along z, Bz=0.3 T, periodic in x and y with 20 images each,
+seeded uniform-random start, in FP32-bulk mode (FP32 FFT/stencil/dE, FP64
+tensor and reductions).  The same workload in plain FP64 and the
+1024x1024x2 film (HBM-resident working set) are reported beside it.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "micromagnetic-gpu-monte-carlo_amd")
+
+METRIC = ("spin-update attempts/s (plain FP64 vs FP32 batched) + HBM GB/s % peak, "
+          "equal accuracy")
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location(
+        "mmc_amd", os.path.join(PKG, "__init__.py"), submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mmc_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def film256(**kw):
+    d = dict(nx=256, ny=256, nz=1, dx=1e-9, dy=1e-9, dz=0.8e-9, pbc_x=20, pbc_y=20,
+             pbc_z=0, ms=1.6e6, aex=16e-12, dind=6e-3, anis_ux=0.0, anis_uy=0.0,
+             anis_uz=1.0, ku1=2e6, bext_x=0.0, bext_y=0.0, bext_z=0.3, cal_demag=1,
+             seed=256)
+    d.update(kw)
+    return d
+
+
+def film1024x2(**kw):
+    d = dict(nx=1024, ny=1024, nz=2, dx=2e-9, dy=2e-9, dz=0.8e-9, pbc_x=10, pbc_y=10,
+             pbc_z=0, ms=1.1e6, aex=15e-12, dind=4e-3, anis_ux=0.0, anis_uy=0.0,
+             anis_uz=1.0, ku1=1e6, bext_x=0.0, bext_y=0.0, bext_z=0.25, cal_demag=1,
+             seed=1024)
+    d.update(kw)
+    return d
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(mod, kw, precision, steps, warmup, T=1400.0, timing=True):
+    """Build, warm up, then time `steps` sweeps with per-kernel HIP events."""
+    c = mod.Context(mod.make_params(**kw), precision)
+    t0 = time.perf_counter()
+    c.build_tensor()
+    c.sync()
+    t_tensor = time.perf_counter() - t0
+    c.init_random_m()
+    c.set_temperature(T)
+    c.sweeps(0, 0, warmup)
+    c.sync()
+    c.set_option("timing", 1 if timing else 0)
+    for k in mod.KINDS:
+        c.kernel_time(k, reset=True)
+    t0 = time.perf_counter()
+    c.sweeps(0, warmup, steps)
+    c.sync()
+    dt = time.perf_counter() - t0
+    kern = {}
+    for k in mod.KINDS:
+        n, ms = c.kernel_time(k, reset=True)
+        if n:
+            kern[k] = dict(launches=n, total_ms=ms, avg_us=1e3 * ms / n,
+                           bytes_per_launch=c.kernel_bytes(k))
+    c.set_option("timing", 0)
+    E = c.energy()
+    N = kw["nx"] * kw["ny"] * kw["nz"]
+    out = dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
+               tensor_build_s=t_tensor, kernels=kern, energy_J=E["total"],
+               m_avg=[float(v) for v in E["m_avg"]], ctx=c)
+    return out
+
+
+def roofline(kern):
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    k = kern[dom]
+    gbs = k["bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9
+    return dom, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                     kernel=dom, avg_us=round(k["avg_us"], 3),
+                     bytes_per_launch=k["bytes_per_launch"])
+
+
+def cpu_baseline(mod, ctx, kw, sweeps):
+    """FP64 oracle sweeps of the same film (tensor taken from the GPU build)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    o = oracle.Oracle(**kw)
+    o.set_tensor(ctx.tensor())
+    o.init_random_m(kw["seed"])
+    o.sweep(1400.0, 0, 0)       # builds the tensor spectrum outside the timed part
+    t0 = time.perf_counter()
+    for s in range(sweeps):
+        o.sweep(1400.0, 0, 1 + s)
+    dt = time.perf_counter() - t0
+    N = kw["nx"] * kw["ny"] * kw["nz"]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=N * sweeps / dt, unit="spin-update attempts/s", cores=cores,
+                kind="port",
+                sample=f"{sweeps} FP64 sweeps of the {kw['nx']}x{kw['ny']}x{kw['nz']} "
+                       f"PBC film on the C oracle (OpenMP), GPU-built tensor")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-sweeps", type=int, default=20)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the FP64 and 1024x1024x2 side measurements")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    mod = load_pkg()
+    mod.lib()
+
+    kw = film256()
+    log(f"[bench] 256x256x1 PBC20 FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
+    r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup)
+    dom, roof = roofline(r32["kernels"])
+    extra = {}
+    if not a.no_extra:
+        log("[bench] same film, plain FP64")
+        r64 = run_gpu(mod, kw, mod.FP64, max(100, a.steps // 4), max(20, a.warmup // 4))
+        extra["fp64_attempts_per_s"] = r64["attempts_per_s"]
+        extra["fp64_ms_per_sweep"] = r64["ms_per_sweep"]
+        extra["fp64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r64["kernels"].items()}
+        extra["fp32_over_fp64"] = r32["attempts_per_s"] / r64["attempts_per_s"]
+        r64["ctx"].close()
+        kw4 = film1024x2()
+        log("[bench] 1024x1024x2 PBC10 FP32-bulk")
+        r4 = run_gpu(mod, kw4, mod.FP32, 200, 20)
+        d4, roof4 = roofline(r4["kernels"])
+        extra["film1024x2_attempts_per_s"] = r4["attempts_per_s"]
+        extra["film1024x2_ms_per_sweep"] = r4["ms_per_sweep"]
+        extra["film1024x2_tensor_build_s"] = r4["tensor_build_s"]
+        extra["film1024x2_roofline"] = roof4
+        extra["film1024x2_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r4["kernels"].items()}
+        r4["ctx"].close()
+    log("[bench] CPU baseline (oracle)")
+    cpu = cpu_baseline(mod, r32["ctx"], kw, a.cpu_sweeps)
+    line = {
+        "metric": METRIC,
+        "value": r32["attempts_per_s"],
+        "unit": "spin-update attempts/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r32["ms_per_sweep"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "anneal sweep, 256x256x1 film, PBC 20x20 images, "
+                               "FP32-bulk, T=1400 K",
+                   "cells": kw["nx"] * kw["ny"] * kw["nz"], "step": "one sweep",
+                   "precision": "fp32-bulk"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_us": {k: round(v["avg_us"], 3) for k, v in r32["kernels"].items()},
+        "tensor_build_s": r32["tensor_build_s"],
+    }
+    line.update(extra)
+    r32["ctx"].close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

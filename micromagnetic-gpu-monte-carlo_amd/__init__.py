@@ -65,7 +65,44 @@ SNAPSHOT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_long, C.c_long,
                           C.POINTER(C.c_double), C.POINTER(C.c_double),
                           C.POINTER(C.c_double))
 
-# every entry point declared in include/mmc.h
+
+class OvfInfo(C.Structure):
+    """``mmc_ovf_info`` (include/mmc_io.h)."""
+
+    _fields_ = [("xnodes", C.c_int), ("ynodes", C.c_int), ("znodes", C.c_int),
+                ("xstepsize", C.c_double), ("ystepsize", C.c_double),
+                ("zstepsize", C.c_double)]
+
+
+class Config(C.Structure):
+    """``mmc_config`` (include/mmc_io.h): every ini key."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("Bextx", C.c_double), ("Bexty", C.c_double), ("Bextz", C.c_double),
+        ("pbc_x", C.c_int), ("pbc_y", C.c_int), ("pbc_z", C.c_int),
+        ("ms", C.c_double), ("Aex", C.c_double), ("Dind", C.c_double),
+        ("anisUx", C.c_double), ("anisUy", C.c_double), ("anisUz", C.c_double),
+        ("Ku1", C.c_double),
+        ("use_random_init", C.c_int),
+        ("randomseed", C.c_longlong),
+        ("ovf_filename", C.c_char * 4096),
+        ("Temperature_start", C.c_double), ("Temperature_end", C.c_double),
+        ("Temperature_step", C.c_double),
+        ("Temperature_use_exp", C.c_int),
+        ("circle_per_stage", C.c_long),
+        ("terminal_output_period", C.c_long),
+        ("energy_output_period", C.c_long),
+        ("ms_output_period", C.c_long),
+        ("rand_block_size", C.c_int),
+        ("cal_demag_flag", C.c_int),
+        ("precision", C.c_int),
+        ("ovf_format", C.c_int),
+    ]
+
+
+# every entry point declared in include/mmc.h and include/mmc_io.h
 EXPORTS = (
     "mmc_create", "mmc_batch_create", "mmc_destroy", "mmc_last_error",
     "mmc_dims", "mmc_count", "mmc_build_tensor", "mmc_set_tensor",
@@ -73,6 +110,7 @@ EXPORTS = (
     "mmc_init_random_m", "mmc_field", "mmc_energy", "mmc_set_temperature",
     "mmc_sweeps", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
     "mmc_kernel_time", "mmc_kernel_bytes",
+    "mmc_ovf_read", "mmc_ovf_write", "mmc_ovf_mesh_equal", "mmc_ini_parse",
 )
 
 
@@ -138,12 +176,54 @@ def lib() -> C.CDLL:
     L.mmc_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_long), dp,
                                   C.c_int]
     L.mmc_kernel_bytes.argtypes = [vp, C.c_char_p, dp]
+    L.mmc_ovf_read.argtypes = [C.c_char_p, C.POINTER(OvfInfo), dp, dp, dp, C.c_long,
+                               C.c_char_p, C.c_int]
+    L.mmc_ovf_write.argtypes = [C.c_char_p, C.POINTER(OvfInfo), dp, dp, dp, C.c_int,
+                                C.c_char_p]
+    L.mmc_ovf_mesh_equal.argtypes = [C.POINTER(OvfInfo), C.POINTER(OvfInfo)]
+    L.mmc_ini_parse.argtypes = [C.c_char_p, C.POINTER(Config), C.c_char_p, C.c_int]
     _lib = L
     return L
 
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ini_parse(path: str) -> Config:
+    cfg = Config()
+    msg = C.create_string_buffer(1024)
+    rc = lib().mmc_ini_parse(path.encode(), C.byref(cfg), msg, 1024)
+    if rc != 0:
+        raise MMCError(f"ini: {msg.value.decode()}")
+    return cfg
+
+
+def ovf_read(path: str):
+    """Returns (OvfInfo, m) with m of shape (3, znodes, ynodes, xnodes)."""
+    L = lib()
+    info = OvfInfo()
+    msg = C.create_string_buffer(1024)
+    rc = L.mmc_ovf_read(path.encode(), C.byref(info), None, None, None, 0, msg, 1024)
+    if rc != 0:
+        raise MMCError(f"ovf: {msg.value.decode()}")
+    n = info.xnodes * info.ynodes * info.znodes
+    m = np.empty((3, info.znodes, info.ynodes, info.xnodes))
+    rc = L.mmc_ovf_read(path.encode(), C.byref(info), _dp(m[0]), _dp(m[1]), _dp(m[2]), n,
+                        msg, 1024)
+    if rc != 0:
+        raise MMCError(f"ovf: {msg.value.decode()}")
+    return info, m
+
+
+def ovf_write(path: str, m: np.ndarray, steps, fmt: int = 8, title: str = "m"):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    _, nz, ny, nx = m.shape
+    info = OvfInfo(nx, ny, nz, *steps)
+    rc = lib().mmc_ovf_write(path.encode(), C.byref(info), _dp(m[0]), _dp(m[1]), _dp(m[2]),
+                             fmt, title.encode())
+    if rc != 0:
+        raise MMCError(f"ovf write failed ({rc})")
 
 
 class Context:

@@ -54,6 +54,8 @@ def lib():
         dp = C.POINTER(C.c_double)
         L.orc_padded_len.argtypes = [C.c_int, C.c_int]
         L.orc_padded_len.restype = C.c_int
+        L.orc_tensor_offset.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_int)]
+        L.orc_tensor_offset.restype = C.c_int
         L.orc_newell_f.argtypes = [C.c_double] * 3
         L.orc_newell_f.restype = C.c_double
         L.orc_newell_g.argtypes = [C.c_double] * 3

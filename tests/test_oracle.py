@@ -1,0 +1,443 @@
+"""Pins of the FP64 oracle (CPU only).
+
+The reference ships no tests, golden vectors or fixtures, and its hot-path
+code cannot be built here, so each oracle function is pinned
+by something independent of its own text: published known-answer vectors,
+numpy's FFT, closed forms, a brute-force quadrature of the dipole kernel,
+symmetry and sum rules, and Boltzmann statistics.  Each check is chosen so a
+plausible slip (dropped term, wrong sign, swapped argument, wrong factor of
+two) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import film, random_m
+
+PI_TRUNC = 3.14159265358
+MU0 = 4e-7 * PI_TRUNC
+KB = 1.3806488e-23
+
+
+# --------------------------------------------------------------------------
+# Philox4x32-10 and the uniform map
+# --------------------------------------------------------------------------
+def test_philox_known_answers(orc):
+    """Random123 published KAT vectors (also reproduced by rocRAND's
+    host-callable philox4x32_10 engine)."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((0xffffffff,) * 4, (0xffffffff,) * 2,
+         (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in kat:
+        assert tuple(orc.philox(ctr, key)) == want
+
+
+def test_u01_range_and_exactness(orc):
+    L = orc.lib()
+    assert L.orc_u01(0) == 2.0 ** -24
+    assert L.orc_u01(0xffffffff) == 1.0 - 2.0 ** -24
+    # representable in FP32: round trip through float32 is exact
+    for r in (1, 12345, 0x80000000, 0xdeadbeef):
+        u = L.orc_u01(r)
+        assert float(np.float32(u)) == u
+
+
+# --------------------------------------------------------------------------
+# FFT
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 13, 30, 49, 64, 96,
+                               100, 127, 192, 256, 384, 1000])
+def test_fft_matches_numpy(orc, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(orc.fft1d(x) - ref).max() / scale < 1e-14
+    assert np.abs(orc.fft1d(x, inverse=True) / n - np.fft.ifft(x)).max() < 1e-15
+
+
+# --------------------------------------------------------------------------
+# padding and the tensor index map
+# --------------------------------------------------------------------------
+def test_padding_rule(orc):
+    L = orc.lib()
+    assert [L.orc_padded_len(n, 0) for n in (1, 2, 16, 64, 96, 128, 192, 256, 1024)] == \
+        [1, 3, 32, 128, 192, 256, 384, 512, 2048]
+    for n in (1, 5, 256, 1024):
+        assert L.orc_padded_len(n, 3) == n
+        assert L.orc_padded_len(n, -1) == n
+    for n in range(1, 300):
+        P = L.orc_padded_len(n, 0)
+        assert P >= 2 * n - 1
+        v = P
+        for f in (2, 3, 5):
+            while v % f == 0:
+                v //= f
+        assert v == 1
+
+
+def _offsets(orc, n, P, pbc):
+    import ctypes as C
+    out = []
+    for i in range(P):
+        o = C.c_int()
+        used = orc.lib().orc_tensor_offset(i, n, P, pbc, C.byref(o))
+        out.append(o.value if used else None)
+    return out
+
+
+def test_tensor_offset_map(orc):
+    # open axis: every offset -(n-1)..(n-1) exactly once, the rest unused
+    for n in (1, 2, 7, 64):
+        P = orc.lib().orc_padded_len(n, 0)
+        offs = [o for o in _offsets(orc, n, P, 0) if o is not None]
+        assert sorted(offs) == list(range(-(n - 1), n))
+    # periodic axis: the reference's map (src/demag_cu.cu:149-151) for n=256
+    # covers -127..128 (asymmetric Nyquist slot), n=7 covers -3..3
+    assert sorted(_offsets(orc, 256, 256, 1)) == list(range(-127, 129))
+    assert sorted(_offsets(orc, 7, 7, 1)) == list(range(-3, 4))
+
+
+# --------------------------------------------------------------------------
+# Newell tensor
+# --------------------------------------------------------------------------
+def test_self_tensor_trace_and_cube(orc):
+    """trace K(0) = -1 for any cell shape (times pi/pi_trunc from the
+    reference's truncated pi in the normalisation, src/demag_cu.cu:221);
+    a cube has K(0) = -1/3 I."""
+    ratio = math.pi / PI_TRUNC
+    for d in ((1e-9, 1e-9, 1e-9), (1e-9, 2e-9, 0.8e-9), (3e-9, 1e-9, 0.5e-9)):
+        o = orc.Oracle(nx=1, ny=1, nz=1, dx=d[0], dy=d[1], dz=d[2])
+        o.build_tensor()
+        K0 = o.mirror()
+        assert abs(K0[0] + K0[3] + K0[5] + ratio) < 1e-12
+        assert np.allclose(K0[[1, 2, 4]], 0.0, atol=1e-14)
+    o = orc.Oracle(nx=1, ny=1, nz=1, dx=2e-9, dy=2e-9, dz=2e-9)
+    o.build_tensor()
+    assert np.allclose(o.mirror()[[0, 3, 5]], -ratio / 3, rtol=0, atol=1e-13)
+
+
+def _dipole_tensor_quadrature(r, d, npts=8):
+    """Cell-averaged dipole field tensor between two cells (centres offset by
+    r, sizes d) by Gauss-Legendre quadrature of (3 r r - I |r|^2)/(4 pi |r|^5)
+    over both cells -> field per unit M at the target cell's average."""
+    g, w = np.polynomial.legendre.leggauss(npts)
+    g = g / 2
+    w = w / 2
+    pts = [np.outer(np.ones(1), g * d[a]).ravel() for a in range(3)]
+    # relative positions: (r + (u - v)) with u, v in each cell
+    acc = np.zeros((3, 3))
+    U = np.stack(np.meshgrid(pts[0], pts[1], pts[2], indexing="ij"), -1).reshape(-1, 3)
+    WU = np.einsum("i,j,k->ijk", w, w, w).ravel()
+    for v, wv in zip(U, WU):
+        R = r + U - v
+        n2 = (R * R).sum(1)
+        n = np.sqrt(n2)
+        T = (3 * R[:, :, None] * R[:, None, :] - np.eye(3) * n2[:, None, None]) / \
+            (4 * np.pi * n[:, None, None] ** 5)
+        acc += wv * np.einsum("i,ijk->jk", WU, T)
+    return acc * d[0] * d[1] * d[2]
+
+
+def test_tensor_against_dipole_quadrature(orc):
+    """Separated cells: the Newell sum equals the brute-force volume
+    integral of the point-dipole kernel (pins f, g, the argument
+    permutations of each component and the -4 pi V normalisation sign)."""
+    d = np.array([1e-9, 1.2e-9, 0.8e-9])
+    o = orc.Oracle(nx=5, ny=5, nz=4, dx=d[0], dy=d[1], dz=d[2])
+    o.build_tensor()
+    K = o.tensor()
+    Px, Py, Pz = o.P
+    comp = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 1): 3, (1, 2): 4, (2, 2): 5}
+    for off in ((3, 1, 2), (-4, 2, 1), (2, -3, -3), (4, 4, 0)):
+        r = np.array(off) * d
+        ref = _dipole_tensor_quadrature(r, d)
+        ix, iy, iz = off[0] % Px, off[1] % Py, off[2] % Pz
+        got = np.zeros((3, 3))
+        for (a, b), k in comp.items():
+            got[a, b] = got[b, a] = K[k, iz, iy, ix] * math.pi / PI_TRUNC
+        assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max(), off
+
+
+def test_tensor_symmetries(orc):
+    o = orc.Oracle(nx=5, ny=5, nz=4, dx=1e-9, dy=1e-9, dz=1e-9)
+    o.build_tensor()
+    K = o.tensor()
+    Px, Py, Pz = o.P
+
+    def at(k, x, y, z):
+        return K[k, z % Pz, y % Py, x % Px]
+
+    # parity of each component under a sign flip of x, y, z
+    parity = {0: (1, 1, 1), 1: (-1, -1, 1), 2: (-1, 1, -1), 3: (1, 1, 1),
+              4: (1, -1, -1), 5: (1, 1, 1)}
+    for k, (sx, sy, sz) in parity.items():
+        for x, y, z in ((1, 2, 1), (3, 1, 2), (4, 4, 0)):
+            v = at(k, x, y, z)
+            assert abs(at(k, -x, y, z) - sx * v) < 1e-12
+            assert abs(at(k, x, -y, z) - sy * v) < 1e-12
+            assert abs(at(k, x, y, -z) - sz * v) < 1e-12
+    # cubic cells: Kyy(a,b,c) = Kxx(b,a,c), Kzz(a,b,c) = Kxx(c,b,a),
+    # Kyz(a,b,c) = Kxy(c,b,a)... (component permutations of src/demag_cu.cu:190-207)
+    for a, b, c in ((1, 2, 0), (2, 1, 1), (3, 0, 2)):
+        assert abs(at(3, a, b, c) - at(0, b, a, c)) < 1e-12
+        assert abs(at(5, a, b, c) - at(0, c, b, a)) < 1e-12
+        assert abs(at(4, a, b, c) - at(1, c, b, a)) < 1e-12
+        assert abs(at(2, a, b, c) - at(1, a, c, b)) < 1e-12
+
+
+def test_thin_film_image_sum_rule(orc):
+    """Uniform in-plane-periodic film: sum_r Kzz(r) -> -1 and
+    sum_r Kxx -> 0 as the number of images grows."""
+    errs = []
+    for p in (2, 8):
+        o = orc.Oracle(nx=8, ny=8, nz=1, pbc_x=p, pbc_y=p)
+        o.build_tensor()
+        K = o.tensor()
+        s = K.reshape(6, -1).sum(1) * math.pi / PI_TRUNC
+        errs.append(abs(s[5] + 1))
+        assert abs(s[0] - s[3]) < 1e-9            # x <-> y symmetry of the square film
+        assert abs(s[0] + s[3] + s[5] + 1) < 1e-6  # trace rule of the full sum
+        assert np.allclose(s[[2, 4]], 0, atol=1e-11)
+        # xy: nonzero only through the one-sided Nyquist image slot of an
+        # even periodic axis (offset +n/2 summed over images -p..p)
+        assert abs(s[1]) < 1e-4
+    assert errs[1] < errs[0] and errs[1] < 0.01
+
+
+def test_pbc_negative_single_image(orc):
+    """pbc<0: periodic axis, zero image range -> same as pbc=0 images but
+    unpadded (src/main.cu:254-260, :475-477)."""
+    o1 = orc.Oracle(nx=6, ny=4, nz=1, pbc_x=-1)
+    o1.build_tensor()
+    o0 = orc.Oracle(nx=6, ny=4, nz=1, pbc_x=0)
+    o0.build_tensor()
+    assert o1.P[0] == 6 and o0.P[0] == orc.lib().orc_padded_len(6, 0)
+    K1, K0 = o1.tensor(), o0.tensor()
+    for ox in range(-2, 3):
+        assert np.allclose(K1[:, 0, :, ox % 6], K0[:, 0, :, ox % o0.P[0]], atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# demag convolution
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("geo", [dict(nx=7, ny=5, nz=1), dict(nx=6, ny=5, nz=3, pbc_x=2),
+                                 dict(nx=4, ny=4, nz=4, pbc_x=1, pbc_y=1, pbc_z=1)])
+def test_demag_fft_equals_direct_sum(orc, geo):
+    o = orc.Oracle(**film(**geo))
+    o.build_tensor()
+    o.set_m(random_m((geo["nx"], geo["ny"], geo["nz"]), 4))
+    assert np.abs(o.demag() - o.demag(direct=True)).max() < 1e-14
+
+
+# --------------------------------------------------------------------------
+# local fields and energies: closed forms
+# --------------------------------------------------------------------------
+def _uniform(n, v):
+    m = np.zeros((3,) + tuple(reversed(n)))
+    for k in range(3):
+        m[k] = v[k]
+    return m
+
+
+def test_uniform_state_closed_form(orc):
+    kw = film(nx=16, ny=16, nz=1, pbc_x=4, pbc_y=4, cal_demag=0, bext_z=0.3)
+    o = orc.Oracle(**kw)
+    o.set_m(_uniform((16, 16, 1), (0, 0, 1)))
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    N = 256
+    E = o.energy()
+    assert abs(E["exch"]) < 1e-30 and abs(E["dmi"]) < 1e-30
+    assert abs(E["anis"] + kw["ku1"] * V * N) < 1e-12 * kw["ku1"] * V * N
+    assert abs(E["zeeman"] + kw["ms"] * 0.3 * V * N) < 1e-12 * kw["ms"] * 0.3 * V * N
+    H = o.field(4 | 8)
+    assert np.allclose(H[2], 2 * kw["ku1"] / (MU0 * kw["ms"]) + 0.3 / MU0, rtol=1e-14)
+    # anisotropy axis is normalised
+    o2 = orc.Oracle(**dict(kw, anis_uz=2.0))
+    o2.set_m(_uniform((16, 16, 1), (0, 0, 1)))
+    assert abs(o2.energy()["anis"] - E["anis"]) < 1e-14 * abs(E["anis"])
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("k", [1, 3, -2])
+def test_helix_exchange_and_dmi(orc, axis, k):
+    """Discrete spin spiral under PBC: e_ex = 2A(1-cos q d)/d^2 and
+    e_dmi = -D sin(q d)/d per cell (interfacial DMI, normal z)."""
+    n = 24
+    kw = film(nx=n if axis == "x" else 4, ny=n if axis == "y" else 4, nz=1,
+              pbc_x=1, pbc_y=1, cal_demag=0, ku1=0.0, bext_z=0.0, dy=1.3e-9)
+    d = kw["dx"] if axis == "x" else kw["dy"]
+    q = 2 * np.pi * k / (n * d)
+    o = orc.Oracle(**kw)
+    nx, ny = kw["nx"], kw["ny"]
+    m = np.zeros((3, 1, ny, nx))
+    pos = (np.arange(nx)[None, :] * kw["dx"]) if axis == "x" else (np.arange(ny)[:, None] * kw["dy"])
+    th = q * pos * np.ones((ny, nx))
+    if axis == "x":
+        m[0, 0], m[2, 0] = np.cos(th), np.sin(th)
+    else:
+        m[1, 0], m[2, 0] = np.cos(th), np.sin(th)
+    o.set_m(m)
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    N = nx * ny
+    E = o.energy()
+    e_ex = 2 * kw["aex"] * (1 - np.cos(q * d)) / d ** 2
+    e_dmi = -kw["dind"] * np.sin(q * d) / d
+    assert abs(E["exch"] - e_ex * V * N) <= 1e-10 * e_ex * V * N
+    assert abs(E["dmi"] - e_dmi * V * N) <= 1e-10 * abs(e_dmi) * V * N
+
+
+def test_energy_is_field_bilinear(orc):
+    """E_term = -1/2 mu0 Ms V sum m.H_term (bilinear terms), Zeeman without 1/2."""
+    kw = film(nx=9, ny=7, nz=2, pbc_y=2)
+    o = orc.Oracle(**kw)
+    o.build_tensor()
+    m = random_m((9, 7, 2), 8)
+    o.set_m(m)
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    E = o.energy()
+    for name, bit, w in (("exch", 1, 0.5), ("dmi", 2, 0.5), ("anis", 4, 0.5),
+                         ("zeeman", 8, 1.0), ("demag", 16, 0.5)):
+        H = o.field(bit)
+        ref = -w * MU0 * kw["ms"] * V * (m * H).sum()
+        assert abs(E[name] - ref) <= 1e-12 * max(abs(ref), 1e-30), name
+
+
+@pytest.mark.parametrize("geo", [dict(nx=6, ny=6, nz=1), dict(nx=5, ny=4, nz=3, pbc_x=1),
+                                 dict(nx=3, ny=5, nz=1, pbc_x=1, pbc_y=2),
+                                 dict(nx=2, ny=3, nz=1, pbc_x=1, pbc_y=2, cal_demag=0)])
+def test_delta_e_equals_energy_difference(orc, geo):
+    """Single-cell dE (frozen pre-move demag + self-tensor correction) equals
+    the full recomputed energy difference.  (Periodic axes of even length
+    with demag are excluded: see the Nyquist-slot test below.)"""
+    kw = film(**geo)
+    o = orc.Oracle(**kw)
+    o.build_tensor()
+    n = (geo["nx"], geo["ny"], geo["nz"])
+    m = random_m(n, 9)
+    rng = np.random.default_rng(1)
+    N = n[0] * n[1] * n[2]
+    for trial in range(6):
+        o.set_m(m)
+        E0 = o.energy()["total"]
+        h = o.demag()
+        cell = int(rng.integers(N))
+        mn = rng.standard_normal(3)
+        mn /= np.linalg.norm(mn)
+        dE = o.delta_e(h, cell, mn)
+        m2 = m.reshape(3, -1).copy()
+        m2[:, cell] = mn
+        o.set_m(m2)
+        E1 = o.energy()["total"]
+        assert abs(dE - (E1 - E0)) <= 1e-9 * abs(E1 - E0) + 1e-32, (trial, dE, E1 - E0)
+
+
+def test_even_periodic_nyquist_slot_is_one_sided(orc):
+    """The reference's index map (src/demag_cu.cu:149-151) stores offset +n/2
+    in the Nyquist slot of an even periodic axis, image-summed over -p..p.
+    A pair at -n/2 therefore sees K(+n/2, .) and an x-odd component keeps
+    its sign where the true tensor flips it: the demag pair matrix is not
+    symmetric there.  Kept for parity; its size falls with the image count."""
+    o = orc.Oracle(nx=4, ny=3, nz=1, pbc_x=1, pbc_y=1)
+    o.build_tensor()
+    K = o.tensor()
+    # Kxy at (x=n/2, y=+1) and (x=n/2, y=-1): odd in y, so they differ in
+    # sign; a symmetric pair matrix would need them equal
+    a, b = K[1, 0, 1, 2], K[1, 0, 2, 2]
+    assert abs(a + b) < 1e-12 and abs(a) > 1e-4
+
+
+# --------------------------------------------------------------------------
+# colouring
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("geo", [dict(nx=6, ny=4, nz=1), dict(nx=5, ny=4, nz=1, pbc_x=1),
+                                 dict(nx=5, ny=7, nz=3, pbc_x=1, pbc_y=1, pbc_z=1),
+                                 dict(nx=2, ny=3, nz=2, pbc_x=1, pbc_y=1)])
+def test_colouring_is_proper(orc, geo):
+    o = orc.Oracle(**film(**geo))
+    n = (geo["nx"], geo["ny"], geo["nz"])
+    pb = (geo.get("pbc_x", 0), geo.get("pbc_y", 0), geo.get("pbc_z", 0))
+    K = o.num_colors()
+    for z in range(n[2]):
+        for y in range(n[1]):
+            for x in range(n[0]):
+                c = o.cell_color(x, y, z)
+                assert 0 <= c < K
+                for a in range(3):
+                    for s in (-1, 1):
+                        co = [x, y, z]
+                        co[a] += s
+                        if not 0 <= co[a] < n[a]:
+                            if not pb[a]:
+                                continue
+                            co[a] %= n[a]
+                        if co == [x, y, z]:
+                            continue
+                        assert o.cell_color(*co) != c
+
+
+# --------------------------------------------------------------------------
+# Metropolis statistics
+# --------------------------------------------------------------------------
+def _langevin(x):
+    return 1 / np.tanh(x) - 1 / x
+
+
+def test_macrospin_zeeman_langevin(orc):
+    """Independent spins in a field: <m.b> = L(Ms V B / kT)."""
+    kw = film(nx=64, ny=1, nz=1, aex=0.0, dind=0.0, ku1=0.0, cal_demag=0,
+              bext_z=0.2, seed=5)
+    o = orc.Oracle(**kw)
+    o.init_random_m(5)
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    x = 2.5
+    T = kw["ms"] * V * 0.2 / (KB * x)
+    acc = []
+    for s in range(1500):
+        o.sweep(T, 0, s)
+        if s >= 100:
+            acc.append(o.m()[2].mean())
+    assert abs(np.mean(acc) - _langevin(x)) < 0.015
+
+
+def test_macrospin_anisotropy(orc):
+    """E = -Ku V (m.u)^2: <m_z^2> = int c^2 e^{s c^2} / int e^{s c^2}."""
+    kw = film(nx=64, ny=1, nz=1, aex=0.0, dind=0.0, ku1=2e6, cal_demag=0,
+              bext_z=0.0, seed=6)
+    o = orc.Oracle(**kw)
+    o.init_random_m(6)
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    sig = 2.0
+    T = kw["ku1"] * V / (KB * sig)
+    c = np.linspace(-1, 1, 20001)
+    wgt = np.exp(sig * c * c)
+    want = (c * c * wgt).sum() / wgt.sum()
+    acc = []
+    for s in range(1500):
+        o.sweep(T, 0, s)
+        if s >= 100:
+            acc.append((o.m()[2] ** 2).mean())
+    assert abs(np.mean(acc) - want) < 0.015
+
+
+def test_exchange_pair_langevin(orc):
+    """Two spins, one bond 2AV/d^2 (1 - m1.m2): <m1.m2> = L(2AV/(d^2 kT))."""
+    kw = film(nx=2, ny=1, nz=1, dind=0.0, ku1=0.0, cal_demag=0, bext_z=0.0, seed=8)
+    o = orc.Oracle(**kw)
+    o.init_random_m(8)
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    J = 2 * kw["aex"] * V / kw["dx"] ** 2
+    x = 2.0
+    T = J / (KB * x)
+    acc = []
+    for s in range(8000):
+        o.sweep(T, 0, s)
+        if s >= 200:
+            mm = o.m().reshape(3, 2)
+            acc.append((mm[:, 0] * mm[:, 1]).sum())
+    assert abs(np.mean(acc) - _langevin(x)) < 0.03
