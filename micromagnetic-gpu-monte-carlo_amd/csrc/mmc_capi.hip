@@ -91,7 +91,6 @@ struct mmc_ctx {
     double *d_H = nullptr;            // field output staging (largest instance)
     size_t m_elems = 0, a_elems = 0, b_elems = 0, khat_elems = 0, k_elems = 0, ka_elems = 0;
     long maxN = 0;
-    bool b_alias = true;              // every instance has Pz == 1: B is A
 
     List xl, zl, yl, cl, tl, txl, tzl, tyl;   // magnetisation passes, cells, tensor passes
     int max_colors = 2;
@@ -327,8 +326,8 @@ int alloc_all(mmc_ctx *c)
     size_t m = 0, a = 0, b = 0, kh = 0, k = 0, ka = 0, tw = 0;
     std::vector<cpx<float>> tw32;
     std::vector<cpx<double>> tw64;
-    c->b_alias = true;
-    for (auto &in : c->inst) if (in.cal_demag && in.Pz > 1) c->b_alias = false;
+    // A and B share one pool; an instance with Pz == 1 has no z pass and its
+    // B region is its A region
     for (auto &in : c->inst) {
         in.m_off = (long)m;
         in.h_off = (long)m;
@@ -337,11 +336,11 @@ int alloc_all(mmc_ctx *c)
         if (in.cal_demag) {
             in.a_off = (long)a;
             a += (size_t)3 * in.Hx * in.nz * in.ny;
-            if (!c->b_alias) {
-                in.b_off = (long)b;
+            if (in.Pz > 1) {
+                in.b_off = (long)b;           // rebased past the A regions below
                 b += (size_t)3 * in.Hx * in.Pz * in.ny;
             } else {
-                in.b_off = in.a_off;
+                in.b_off = -1;
             }
             in.khat_off = (long)kh;
             kh += (size_t)6 * in.Hx * in.Pz * in.Py;
@@ -354,14 +353,15 @@ int alloc_all(mmc_ctx *c)
             in.twz = (long)tw; twiddles(in.Pz, tw32); twiddles(in.Pz, tw64); tw += in.Pz;
         }
     }
+    for (auto &in : c->inst)
+        if (in.cal_demag) in.b_off = in.b_off < 0 ? in.a_off : (long)a + in.b_off;
     c->m_elems = m; c->a_elems = a; c->b_elems = b; c->khat_elems = kh; c->k_elems = k;
     c->ka_elems = ka;
     HIPCHK(c, hipMalloc(&c->d_m, std::max<size_t>(1, m) * ts));
     HIPCHK(c, hipMalloc(&c->d_h, std::max<size_t>(1, m) * ts));
     HIPCHK(c, hipMemsetAsync(c->d_h, 0, std::max<size_t>(1, m) * ts, c->stream));
-    if (a) HIPCHK(c, hipMalloc(&c->d_a, a * cs));
-    if (b) HIPCHK(c, hipMalloc(&c->d_b, b * cs));
-    if (c->b_alias) c->d_b = c->d_a;
+    if (a + b) HIPCHK(c, hipMalloc(&c->d_a, (a + b) * cs));
+    c->d_b = c->d_a;
     if (kh) HIPCHK(c, hipMalloc(&c->d_khat, kh * cs));
     if (k) {
         HIPCHK(c, hipMalloc(&c->d_K, k * sizeof(double)));
@@ -682,7 +682,6 @@ void mmc_destroy(mmc_ctx *c)
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
         if (p) hipFree(p);
-    if (!c->b_alias && c->d_b) hipFree(c->d_b);
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl})
         if (l->dev) hipFree(l->dev);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -8,7 +8,7 @@
 // HBM layout of one instance (b = sizeof(T), T = float or double):
 //   m, h        [3][nz][ny][nx]            T       magnetisation, demag/Ms
 //   A           [3][Hx][nz][ny]            cpx<T>  x-spectrum, y fastest
-//   B           [3][Hx][Pz][ny]            cpx<T>  xz-spectrum (A if Pz==1)
+//   B           [3][Hx][Pz][ny]            cpx<T>  xz-spectrum (A if Pz==1; same pool)
 //   Khat        [6][Hx][Pz][Py]            cpx<T>  tensor spectrum / (PxPyPz)
 //   K           [6][Pz][Py][Px]            double  real-space tensor
 // with Hx = Px/2 + 1 (half spectrum of the real x transform).
