@@ -92,7 +92,7 @@ struct mmc_ctx {
     size_t m_elems = 0, a_elems = 0, b_elems = 0, khat_elems = 0, k_elems = 0, ka_elems = 0;
     long maxN = 0;
 
-    List xl, zl, yl, cl, tl, txl, tzl, tyl;   // magnetisation passes, cells, tensor passes
+    List xl, zl, yl, cl, ml, tl, txl, tzl, tyl;   // FFT passes, cells, colour slots, tensor passes
     int max_colors = 2;
     bool any_demag = false;
     bool tensor_ready = false;
@@ -270,7 +270,7 @@ void build_lists(mmc_ctx *c)
 {
     const size_t cs = 2 * c->tsize;   // complex element size
     auto reset = [&](List &l) { l.host.clear(); l.first.assign(1, 0); l.lds = 0; };
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
         if (in.cal_demag) {
@@ -306,7 +306,10 @@ void build_lists(mmc_ctx *c)
             for (long s = 0; s < in.NP; s += kThreads) c->tl.host.push_back({i, (int)s, 0, 0});
         }
         for (long s = 0; s < in.N; s += kThreads) c->cl.host.push_back({i, (int)s, 0, 0});
-        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl})
+        // Metropolis: with two colours a pass visits ceil(nx/2) slots per row
+        const long slots = in.ncolors == 2 ? (long)((in.nx + 1) / 2) * in.ny * in.nz : in.N;
+        for (long s = 0; s < slots; s += kThreads) c->ml.host.push_back({i, (int)s, 0, 0});
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl})
             l->first.push_back(l->n());
     }
 }
@@ -384,7 +387,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
     HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -465,7 +468,7 @@ int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
     hipStream_t st = c->stream;
     for (int col = 0; col < c->max_colors; ++col)
         HIPCHK(c, timed(c, K_METRO, [&] {
-                   return launch_metro<T>(st, c->dInst, c->cl.dev, c->cl.n(), (T *)c->d_m,
+                   return launch_metro<T>(st, c->dInst, c->ml.dev, c->ml.n(), (T *)c->d_m,
                                           (const T *)c->d_h, c->d_beta, c->d_ctl, sweep_off, col,
                                           c->any_demag ? 1 : 0);
                }));
@@ -682,7 +685,7 @@ void mmc_destroy(mmc_ctx *c)
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
         if (p) hipFree(p);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->tl, &c->txl, &c->tzl, &c->tyl})
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl})
         if (l->dev) hipFree(l->dev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

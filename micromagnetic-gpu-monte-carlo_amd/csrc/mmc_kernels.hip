@@ -528,16 +528,31 @@ __global__ __launch_bounds__(kThreads) void k_metro(const Inst *__restrict__ I,
 {
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
-    const long i = (long)w.a + threadIdx.x;
-    if (i >= in.N) return;
     const int nx = in.nx, ny = in.ny;
-    const int x = (int)(i % nx);
-    const long r = i / nx;
-    const int y = (int)(r % ny);
-    const int z = (int)(r / ny);
-    const int col = (axis_color(x, nx, in.pbc[0]) + axis_color(y, ny, in.pbc[1]) +
-                     axis_color(z, in.nz, in.pbc[2])) % in.ncolors;
-    if (col != color) return;
+    int x, y, z;
+    long i;
+    if (in.ncolors == 2) {
+        // slot -> the colour's cell in its row: x = 2j + ((y + z + color) & 1)
+        const int h2 = (nx + 1) >> 1;
+        const long sl = (long)w.a + threadIdx.x;
+        const long r = sl / h2;
+        if (r >= (long)ny * in.nz) return;
+        y = (int)(r % ny);
+        z = (int)(r / ny);
+        x = 2 * (int)(sl - r * h2) + ((y + z + color) & 1);
+        if (x >= nx) return;
+        i = x + (long)nx * r;
+    } else {
+        i = (long)w.a + threadIdx.x;
+        if (i >= in.N) return;
+        x = (int)(i % nx);
+        const long r = i / nx;
+        y = (int)(r % ny);
+        z = (int)(r / ny);
+        const int col = (axis_color(x, nx, in.pbc[0]) + axis_color(y, ny, in.pbc[1]) +
+                         axis_color(z, in.nz, in.pbc[2])) % in.ncolors;
+        if (col != color) return;
+    }
     T *m = mb + in.m_off;
     const long N = in.N;
     const V3<T> mi = load_m(m, N, i);
