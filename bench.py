@@ -8,6 +8,10 @@ seeded uniform-random start, in FP32-bulk mode (FP32 FFT/stencil/dE, FP64
 tensor and reductions).  The same workload in plain FP64 and the
 1024x1024x2 film (HBM-resident working set) are reported beside it.
 
+The timed steps replay hipGraph-captured sweeps; the per-kernel durations
+behind `roofline` come from the sweeps right after them, launched eagerly with
+HIP events recorded by each kernel's dispatch packet on the library's stream.
+
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -58,24 +62,42 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(mod, kw, precision, steps, warmup, T=1400.0, timing=True):
-    """Build, warm up, then time `steps` sweeps with per-kernel HIP events."""
-    c = mod.Context(mod.make_params(**kw), precision)
+def make_ctx(mod, kws, precision, T=1400.0):
+    c = mod.Context([mod.make_params(**kw) for kw in kws], precision)
     t0 = time.perf_counter()
     c.build_tensor()
     c.sync()
     t_tensor = time.perf_counter() - t0
     c.init_random_m()
     c.set_temperature(T)
+    return c, t_tensor
+
+
+def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=None):
+    """Warm up, time `steps` sweeps (hipGraph replay when `graph`), then time
+    `ksteps` more sweeps with every launch bracketed by dispatch-packet HIP
+    events for the per-kernel durations."""
+    if isinstance(kws, dict):
+        kws = [kws]
+    c, t_tensor = make_ctx(mod, kws, precision, T)
+    N = sum(kw["nx"] * kw["ny"] * kw["nz"] for kw in kws)
+    if graph:
+        c.set_option("graph", 1)
+        c.set_option("graph_sweeps", 20)
     c.sweeps(0, 0, warmup)
     c.sync()
-    c.set_option("timing", 1 if timing else 0)
-    for k in mod.KINDS:
-        c.kernel_time(k, reset=True)
     t0 = time.perf_counter()
     c.sweeps(0, warmup, steps)
     c.sync()
     dt = time.perf_counter() - t0
+    ksteps = ksteps or min(steps, 400)
+    c.set_option("timing", 1)
+    for k in mod.KINDS:
+        c.kernel_time(k, reset=True)
+    t1 = time.perf_counter()
+    c.sweeps(0, warmup + steps, ksteps)
+    c.sync()
+    dt_eager = time.perf_counter() - t1
     kern = {}
     for k in mod.KINDS:
         n, ms = c.kernel_time(k, reset=True)
@@ -84,10 +106,25 @@ def run_gpu(mod, kw, precision, steps, warmup, T=1400.0, timing=True):
                            bytes_per_launch=c.kernel_bytes(k))
     c.set_option("timing", 0)
     E = c.energy()
-    N = kw["nx"] * kw["ny"] * kw["nz"]
-    out = dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
-               tensor_build_s=t_tensor, kernels=kern, energy_J=E["total"],
-               m_avg=[float(v) for v in E["m_avg"]], ctx=c)
+    return dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
+                eager_timed_attempts_per_s=N * ksteps / dt_eager,
+                tensor_build_s=t_tensor, kernels=kern, energy_J=E["total"], ctx=c, cells=N)
+
+
+def batch_params(count=64, master_seed=2024):
+    """Independent anneals of uneven size: nz=1 grids n x n with n drawn from
+    {64,96,128,192,256}, Bz ~ U(0,0.6) T, Dind ~ U(3,6) mJ/m^2, the 64x64
+    film's Ms/Aex/Ku1, distinct seeds."""
+    import numpy as np
+    rng = np.random.default_rng(master_seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice([64, 96, 128, 192, 256]))
+        out.append(dict(nx=n, ny=n, nz=1, dx=1e-9, dy=1e-9, dz=0.8e-9, pbc_x=0, pbc_y=0,
+                        pbc_z=0, ms=1.6e6, aex=16e-12, dind=float(rng.uniform(3e-3, 6e-3)),
+                        anis_ux=0.0, anis_uy=0.0, anis_uz=1.0, ku1=2e6, bext_x=0.0,
+                        bext_y=0.0, bext_z=float(rng.uniform(0.0, 0.6)), cal_demag=1,
+                        seed=1000 + i))
     return out
 
 
@@ -143,12 +180,29 @@ def main():
     extra = {}
     if not a.no_extra:
         log("[bench] same film, plain FP64")
-        r64 = run_gpu(mod, kw, mod.FP64, max(100, a.steps // 4), max(20, a.warmup // 4))
+        r64 = run_gpu(mod, kw, mod.FP64, max(200, a.steps // 2), max(20, a.warmup // 4))
         extra["fp64_attempts_per_s"] = r64["attempts_per_s"]
         extra["fp64_ms_per_sweep"] = r64["ms_per_sweep"]
         extra["fp64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r64["kernels"].items()}
         extra["fp32_over_fp64"] = r32["attempts_per_s"] / r64["attempts_per_s"]
         r64["ctx"].close()
+        log("[bench] 64x64x1 open film, plain FP64 single instance")
+        kw0 = film256(nx=64, ny=64, pbc_x=0, pbc_y=0)
+        r0 = run_gpu(mod, kw0, mod.FP64, max(400, a.steps), a.warmup)
+        extra["film64_fp64_attempts_per_s"] = r0["attempts_per_s"]
+        extra["film64_fp64_ms_per_sweep"] = r0["ms_per_sweep"]
+        r0["ctx"].close()
+        log("[bench] 64 uneven films batched, FP32-bulk")
+        bp = batch_params()
+        rb = run_gpu(mod, bp, mod.FP32, 200, 20)
+        db, roofb = roofline(rb["kernels"])
+        extra["batch64_fp32_attempts_per_s"] = rb["attempts_per_s"]
+        extra["batch64_ms_per_sweep"] = rb["ms_per_sweep"]
+        extra["batch64_cells"] = rb["cells"]
+        extra["batch64_roofline"] = roofb
+        extra["batch64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in rb["kernels"].items()}
+        extra["batched_fp32_over_single_fp64"] = rb["attempts_per_s"] / r0["attempts_per_s"]
+        rb["ctx"].close()
         kw4 = film1024x2()
         log("[bench] 1024x1024x2 PBC10 FP32-bulk")
         r4 = run_gpu(mod, kw4, mod.FP32, 200, 20)
@@ -181,6 +235,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in r32["kernels"].items()},
+        "eager_timed_attempts_per_s": r32["eager_timed_attempts_per_s"],
         "tensor_build_s": r32["tensor_build_s"],
     }
     line.update(extra)

@@ -183,8 +183,12 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
 /* ---- instrumentation -------------------------------------------------- */
 
 /* Options: "graph" (1 = replay sweeps from a captured hipGraph),
- * "timing" (1 = bracket every launch with hipEvents on the context's
- * stream).  Unknown keys return MMC_EINVAL. */
+ * "graph_sweeps" (sweeps per captured graph, default 16),
+ * "timing" (1 = time every launch with hipEvents recorded by its dispatch
+ * packet on the context's stream; sweeps then launch eagerly),
+ * "tyx" (x-pass rows per workgroup), "ncol" (y-pass columns per workgroup).
+ * Tile options only regroup independent transforms: results are unchanged.
+ * Unknown keys return MMC_EINVAL. */
 int mmc_set_option(mmc_ctx *ctx, const char *key, long value);
 
 /* Launch count and summed device time [ms] of one kernel kind recorded

@@ -133,10 +133,21 @@ __device__ __forceinline__ void fft_stage(const cpx<T> *__restrict__ in,
     const int nb = n / R;
     const int total = nb * nrows;
     const int tws = n / (Ns * R);
+    // power-of-two butterfly counts (every radix-4/2 plan): shifts and masks
+    // instead of integer division
+    const bool p2 = ((nb & (nb - 1)) == 0) && ((Ns & (Ns - 1)) == 0);
+    const int lnb = __builtin_ctz((unsigned)nb);
     for (int t = threadIdx.x; t < total; t += kThreads) {
-        const int row = t / nb;
-        const int j = t - row * nb;
-        const int k = j % Ns;
+        int row, j, k;
+        if (p2) {
+            row = t >> lnb;
+            j = t & (nb - 1);
+            k = j & (Ns - 1);
+        } else {
+            row = t / nb;
+            j = t - row * nb;
+            k = j % Ns;
+        }
         const cpx<T> *ib = in + row * rs;
         cpx<T> v[R];
         #pragma unroll

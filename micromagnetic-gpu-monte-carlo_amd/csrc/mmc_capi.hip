@@ -79,6 +79,7 @@ struct mmc_ctx {
     int device = 0;
     size_t lds_pref = 64 * 1024;      // preferred LDS per FFT workgroup
     size_t lds_max = 64 * 1024;       // device limit per workgroup
+    int num_cu = 256;                 // compute units of the device
 
     // pools
     void *d_m = nullptr, *d_h = nullptr, *d_a = nullptr, *d_b = nullptr, *d_khat = nullptr;
@@ -168,9 +169,9 @@ hipError_t timed(mmc_ctx *c, int kind, F &&f)
     if (c->ev_used + 2 > 4096) harvest(c);
     hipEvent_t a = next_event(c), b = next_event(c);
     if (!a || !b) return f();
-    hipEventRecord(a, c->stream);
+    set_launch_events(a, b);
     hipError_t e = f();
-    hipEventRecord(b, c->stream);
+    set_launch_events(nullptr, nullptr);
     c->recs.push_back({kind, a, b});
     return e;
 }
@@ -187,17 +188,24 @@ int fit_units(int want, size_t bytes_per_unit, size_t pref, size_t maxb)
     return 0;
 }
 
+// The FFT kernels also hold their twiddle table (n complex) in LDS.
+int fit_tw(int want, size_t per_unit, size_t tw, size_t pref, size_t maxb)
+{
+    if (tw >= maxb) return 0;
+    return fit_units(want, per_unit, pref > tw ? pref - tw : 0, maxb - tw);
+}
+
 bool choose_tiles(const mmc_ctx *c, Inst &in, size_t pref, size_t maxb)
 {
     const size_t cs = 2 * c->tsize;
     const int tymax = c->tsize == 4 ? 16 : 8;
-    const int pairs = fit_units(tymax / 2, 2 * (size_t)in.Px * cs, pref, maxb);
+    const int pairs = fit_tw(tymax / 2, 2 * (size_t)in.Px * cs, in.Px * cs, pref, maxb);
     in.tyx = 2 * pairs;
-    in.tyz = fit_units(tymax, 2 * (size_t)in.Pz * cs, pref, maxb);
-    in.ncol = fit_units(4, 2 * 3 * (size_t)in.Py * cs, pref, maxb);
-    in.tyx64 = 2 * fit_units(4, 2 * (size_t)in.Px * 16, pref, maxb);
-    in.tyz64 = fit_units(8, 2 * (size_t)in.Pz * 16, pref, maxb);
-    const bool yt = 2 * (size_t)in.Py * 16 <= maxb;
+    in.tyz = in.Pz <= kZDirect ? 1 : fit_tw(tymax, 2 * (size_t)in.Pz * cs, in.Pz * cs, pref, maxb);
+    in.ncol = fit_tw(4, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, pref, maxb);
+    in.tyx64 = 2 * fit_tw(4, 2 * (size_t)in.Px * 16, in.Px * 16, pref, maxb);
+    in.tyz64 = in.Pz <= kZDirect ? 1 : fit_tw(8, 2 * (size_t)in.Pz * 16, in.Pz * 16, pref, maxb);
+    const bool yt = 3 * (size_t)in.Py * 16 <= maxb;
     return pairs && in.tyz && in.ncol && in.tyx64 && in.tyz64 && yt;
 }
 
@@ -273,36 +281,39 @@ void build_lists(mmc_ctx *c)
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
+        const bool zdirect = in.Pz <= kZDirect;
+        const int zstep = zdirect ? kThreads : in.tyz;
+        const int zstep64 = zdirect ? kThreads : in.tyz64;
         if (in.cal_demag) {
             for (int comp = 0; comp < 3; ++comp)
                 for (int z = 0; z < in.nz; ++z)
                     for (int y0 = 0; y0 < in.ny; y0 += in.tyx) c->xl.host.push_back({i, comp, z, y0});
-            c->xl.lds = std::max(c->xl.lds, 2 * (size_t)(in.tyx / 2) * in.Px * cs);
+            c->xl.lds = std::max(c->xl.lds, (2 * (size_t)(in.tyx / 2) + 1) * in.Px * cs);
             if (in.Pz > 1) {
                 for (int comp = 0; comp < 3; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
-                        for (int y0 = 0; y0 < in.ny; y0 += in.tyz) c->zl.host.push_back({i, comp, kx, y0});
-                c->zl.lds = std::max(c->zl.lds, 2 * (size_t)in.Pz * in.tyz * cs);
+                        for (int y0 = 0; y0 < in.ny; y0 += zstep) c->zl.host.push_back({i, comp, kx, y0});
+                if (!zdirect) c->zl.lds = std::max(c->zl.lds, (2 * (size_t)in.tyz + 1) * in.Pz * cs);
             }
             for (int kz = 0; kz < in.Pz; ++kz)
                 for (int kx0 = 0; kx0 < in.Hx; kx0 += in.ncol)
                     c->yl.host.push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
-            c->yl.lds = std::max(c->yl.lds, 2 * (size_t)3 * in.ncol * in.Py * cs);
+            c->yl.lds = std::max(c->yl.lds, (2 * (size_t)3 * in.ncol + 1) * in.Py * cs);
             // tensor passes (FP64)
             for (int comp = 0; comp < 6; ++comp)
                 for (int z = 0; z < in.Pz; ++z)
                     for (int y0 = 0; y0 < in.Py; y0 += in.tyx64) c->txl.host.push_back({i, comp, z, y0});
-            c->txl.lds = std::max(c->txl.lds, 2 * (size_t)(in.tyx64 / 2) * in.Px * 16);
+            c->txl.lds = std::max(c->txl.lds, (2 * (size_t)(in.tyx64 / 2) + 1) * in.Px * 16);
             if (in.Pz > 1) {
                 for (int comp = 0; comp < 6; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
-                        for (int y0 = 0; y0 < in.Py; y0 += in.tyz64) c->tzl.host.push_back({i, comp, kx, y0});
-                c->tzl.lds = std::max(c->tzl.lds, 2 * (size_t)in.Pz * in.tyz64 * 16);
+                        for (int y0 = 0; y0 < in.Py; y0 += zstep64) c->tzl.host.push_back({i, comp, kx, y0});
+                if (!zdirect) c->tzl.lds = std::max(c->tzl.lds, (2 * (size_t)in.tyz64 + 1) * in.Pz * 16);
             }
             for (int comp = 0; comp < 6; ++comp)
                 for (int kx = 0; kx < in.Hx; ++kx)
                     for (int kz = 0; kz < in.Pz; ++kz) c->tyl.host.push_back({i, comp, kx, kz});
-            c->tyl.lds = std::max(c->tyl.lds, 2 * (size_t)in.Py * 16);
+            c->tyl.lds = std::max(c->tyl.lds, 3 * (size_t)in.Py * 16);
             for (long s = 0; s < in.NP; s += kThreads) c->tl.host.push_back({i, (int)s, 0, 0});
         }
         for (long s = 0; s < in.N; s += kThreads) c->cl.host.push_back({i, (int)s, 0, 0});
@@ -320,6 +331,50 @@ void twiddles(int n, std::vector<cpx<T>> &out)
     for (int t = 0; t < n; ++t) {
         long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
         out.push_back({(T)cosl(a), (T)sinl(a)});
+    }
+}
+
+// Largest x-pass row tile (even, >= 2) and y-pass column count that fit the
+// LDS limits of instance `in`.
+int max_tyx(const mmc_ctx *c, const Inst &in)
+{
+    const size_t cs = 2 * c->tsize;
+    return 2 * fit_tw(64, 2 * (size_t)in.Px * cs, in.Px * cs, c->lds_max, c->lds_max);
+}
+
+int max_ncol(const mmc_ctx *c, const Inst &in)
+{
+    const size_t cs = 2 * c->tsize;
+    return fit_tw(64, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, c->lds_max, c->lds_max);
+}
+
+// Small grids cannot fill the chip with wide tiles: halve the x-pass row
+// tiles and y-pass column groups until each pass has about two workgroups
+// per compute unit (or the tiles are minimal).
+void adapt_tiles(mmc_ctx *c)
+{
+    const long target = 2L * c->num_cu;
+    auto xitems = [&] {
+        long n = 0;
+        for (auto &in : c->inst)
+            if (in.cal_demag) n += 3L * in.nz * ((in.ny + in.tyx - 1) / in.tyx);
+        return n;
+    };
+    auto yitems = [&] {
+        long n = 0;
+        for (auto &in : c->inst)
+            if (in.cal_demag) n += (long)in.Pz * ((in.Hx + in.ncol - 1) / in.ncol);
+        return n;
+    };
+    for (bool changed = true; changed && xitems() < target;) {
+        changed = false;
+        for (auto &in : c->inst)
+            if (in.tyx > 2) { in.tyx = std::max(2, (in.tyx / 2) & ~1); changed = true; }
+    }
+    for (bool changed = true; changed && yitems() < target;) {
+        changed = false;
+        for (auto &in : c->inst)
+            if (in.ncol > 1) { in.ncol = std::max(1, in.ncol / 2); changed = true; }
     }
 }
 
@@ -421,6 +476,9 @@ int create_common(const mmc_params *p, int count, int precision, mmc_ctx **out)
             if (c->lds_max > 64 * 1024 && set_lds_limit(c->lds_max) != hipSuccess) c->lds_max = 64 * 1024;
         }
         c->lds_pref = std::min(c->lds_pref, c->lds_max);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
+            v > 0)
+            c->num_cu = v;
     }
     for (int i = 0; i < count; ++i) {
         if ((rc = setup_inst(c, i, p[i], c->inst[i]))) { *out = c; return rc; }
@@ -431,10 +489,31 @@ int create_common(const mmc_params *p, int count, int precision, mmc_ctx **out)
         *out = c;
         return fail(c, MMC_EHIP, "hipStreamCreate failed");
     }
+    adapt_tiles(c);
     build_lists(c);
     rc = alloc_all(c);
     *out = c;
     return rc;
+}
+
+// Set every instance's x-pass row tile / y-pass column group (clamped to
+// what LDS allows), rebuild and upload the work lists, drop any graph.
+int retile(mmc_ctx *c, int tyx, int ncol)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &in : c->inst) {
+        if (tyx > 0) in.tyx = std::max(2, std::min(tyx & ~1, max_tyx(c, in)));
+        if (ncol > 0) in.ncol = std::max(1, std::min(ncol, max_ncol(c, in)));
+    }
+    build_lists(c);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+        int rc = upload_list(c, *l);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    return MMC_OK;
 }
 
 // ---- demag chain -------------------------------------------------------------
@@ -1010,6 +1089,10 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
         c->graph_sweeps = (int)value;
         return MMC_OK;
+    }
+    if (k == "tyx" || k == "ncol") {
+        if (value < 1 || value > 4096) return fail(c, MMC_EINVAL, "%s must be in [1,4096]", key);
+        return k == "tyx" ? retile(c, (int)value, 0) : retile(c, 0, (int)value);
     }
     if (k == "timing") {
         if (!value) harvest(c);

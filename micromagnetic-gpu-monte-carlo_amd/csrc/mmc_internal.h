@@ -23,6 +23,7 @@ namespace mmc {
 
 constexpr int kThreads = 256;      // threads per workgroup (4 wave64)
 constexpr int kMaxStages = 16;     // radix stages per FFT plan
+constexpr int kZDirect = 8;        // z lengths up to this use a direct DFT
 
 template <typename T>
 struct alignas(2 * sizeof(T)) cpx {
@@ -120,5 +121,9 @@ hipError_t launch_set_ctl(hipStream_t st, uint32_t *ctl, uint32_t stage,
                           uint32_t sweep_base);
 // Allow `bytes` of dynamic LDS on every FFT kernel (above the 64 KB default).
 hipError_t set_lds_limit(size_t bytes);
+// Events for the next launch of this thread: the dispatch packet itself
+// records a (start) and b (end), so the measured span is the kernel alone.
+// Reset after every launch wrapper call; (nullptr, nullptr) = untimed.
+void set_launch_events(hipEvent_t a, hipEvent_t b);
 
 }  // namespace mmc

@@ -20,6 +20,8 @@
 #include "fft_lds.h"
 #include "mmc_internal.h"
 
+#include <hip/hip_ext.h>
+
 namespace mmc {
 
 // ============================================================================
@@ -212,6 +214,16 @@ __global__ __launch_bounds__(kThreads) void k_tensor(const Inst *__restrict__ I,
 // ============================================================================
 extern __shared__ __align__(16) unsigned char g_smem[];
 
+// Copy a twiddle table into LDS (after the two ping-pong buffers); callers
+// synchronise once data and twiddles are both loaded.
+template <typename T>
+__device__ __forceinline__ const cpx<T> *lds_twiddles(cpx<T> *dst, const cpx<T> *__restrict__ src,
+                                                      int n)
+{
+    for (int e = threadIdx.x; e < n; e += kThreads) dst[e] = src[e];
+    return dst;
+}
+
 // mode 0: magnetisation (m -> A), extents (nx, ny, nz), 3 components
 // mode 1: tensor (K -> KA, FP64), extents (Px, Py, Pz), 6 components
 template <typename T>
@@ -240,6 +252,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd(const Inst *__restrict__ I,
     }
     src += (long)z * Lx * Ly;
     cpx<T> *s1 = s + (ty >> 1) * P;
+    const cpx<T> *tw = lds_twiddles(s1 + (ty >> 1) * P, twb + in.twx, P);
     const int nyt = min(ty, Ly - y0);
     const int npair = (nyt + 1) >> 1;
     for (int e = threadIdx.x; e < npair * P; e += kThreads) {
@@ -253,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd(const Inst *__restrict__ I,
         s[q * P + x] = {re, im};
     }
     __syncthreads();
-    s = fft_rows<T>(s, s1, npair, 1, P, in.plx, twb + in.twx);
+    s = fft_rows<T>(s, s1, npair, 1, P, in.plx, tw);
     // split the packed spectrum: Xa = (Z_k + conj Z_-k)/2, Xb = -i(Z_k - conj Z_-k)/2
     for (int e = threadIdx.x; e < H * npair; e += kThreads) {
         const int k = e / npair, q = e - k * npair;
@@ -279,6 +292,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
     const int P = in.Px, H = in.Hx;
     const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
     cpx<T> *s1 = s + (in.tyx >> 1) * P;
+    const cpx<T> *tw = lds_twiddles(s1 + (in.tyx >> 1) * P, twb + in.twx, P);
     const cpx<T> *src = specb + in.a_off + (long)c * H * Lz * Ly;
     T *dst = realb + in.h_off + (long)c * in.N + (long)z * Lx * Ly;
     const int nyt = min(in.tyx, Ly - y0);
@@ -294,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
         if (k > 0 && P - k >= H) s[q * P + P - k] = {a.x + b.y, a.y - b.x};
     }
     __syncthreads();
-    s = fft_rows<T>(s, s1, npair, 1, P, in.plx, twb + in.twx);
+    s = fft_rows<T>(s, s1, npair, 1, P, in.plx, tw);
     for (int e = threadIdx.x; e < nyt * Lx; e += kThreads) {
         const int yy = e / Lx, x = e - yy * Lx;
         const cpx<T> v = s[(yy >> 1) * P + x];
@@ -302,7 +316,40 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
     }
 }
 
-// z transform: item (inst, c, kx, y0).  LDS element (y, z) at s[z*nyt + y].
+// Direct DFT along z of one y column (element z at p[z*stride]): nin inputs
+// (zero above), nout outputs, length P <= kZDirect, forward or inverse
+// (conjugate twiddles).  In place safe: all inputs are read first.
+template <typename T, bool INV>
+__device__ __forceinline__ void zdft_column(const cpx<T> *src, cpx<T> *dst, int stride,
+                                            int nin, int nout, int P,
+                                            const cpx<T> *__restrict__ tw)
+{
+    cpx<T> v[kZDirect];
+    #pragma unroll
+    for (int z = 0; z < kZDirect; ++z)
+        v[z] = z < nin ? src[(long)z * stride] : cpx<T>{T(0), T(0)};
+    cpx<T> out[kZDirect];
+    #pragma unroll
+    for (int k = 0; k < kZDirect; ++k) {
+        cpx<T> acc = v[0];
+        #pragma unroll
+        for (int z = 1; z < kZDirect; ++z) {
+            if (z < nin && k < nout) {
+                cpx<T> w = tw[(k * z) % P];
+                if (INV) w.y = -w.y;
+                acc = cadd(acc, cmul(v[z], w));
+            }
+        }
+        out[k] = acc;
+    }
+    #pragma unroll
+    for (int k = 0; k < kZDirect; ++k)
+        if (k < nout) dst[(long)k * stride] = out[k];
+}
+
+// z transform: item (inst, c, kx, y0).  Short z (Pz <= kZDirect): one thread
+// per y column, direct DFT in registers, item covers kThreads y values.
+// Otherwise LDS Stockham with element (y, z) at s[z*nyt + y].
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_zfwd(const Inst *__restrict__ I,
                                                    const Work *__restrict__ W, int mode,
@@ -327,14 +374,21 @@ __global__ __launch_bounds__(kThreads) void k_zfwd(const Inst *__restrict__ I,
         src = srcb + in.ka_off + ((long)c * H + kx) * Lz * Ly;
         dst = dstb + in.ka_off + ((long)c * H + kx) * Pz * Ly;
     }
+    if (Pz <= kZDirect) {   // short z: direct DFT of one y column per thread
+        const int y = y0 + threadIdx.x;
+        if (y >= Ly) return;
+        zdft_column<T, false>(src + y, dst + y, Ly, Lz, Pz, Pz, twb + in.twz);
+        return;
+    }
     cpx<T> *s1 = s + Pz * ty;
+    const cpx<T> *tw = lds_twiddles(s1 + Pz * ty, twb + in.twz, Pz);
     const int nyt = min(ty, Ly - y0);
     for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
         const int zz = e / nyt, yy = e - zz * nyt;
         s[e] = zz < Lz ? src[(long)zz * Ly + y0 + yy] : cpx<T>{T(0), T(0)};
     }
     __syncthreads();
-    s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, twb + in.twz);
+    s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, tw);
     for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
         const int zz = e / nyt, yy = e - zz * nyt;
         dst[(long)zz * Ly + y0 + yy] = s[e];
@@ -356,13 +410,20 @@ __global__ __launch_bounds__(kThreads) void k_zinv(const Inst *__restrict__ I,
     cpx<T> *s1 = s + Pz * in.tyz;
     const cpx<T> *src = srcb + in.b_off + ((long)c * H + kx) * Pz * Ly;
     cpx<T> *dst = dstb + in.a_off + ((long)c * H + kx) * Lz * Ly;
+    if (Pz <= kZDirect) {
+        const int y = y0 + threadIdx.x;
+        if (y >= Ly) return;
+        zdft_column<T, true>(src + y, dst + y, Ly, Pz, Lz, Pz, twb + in.twz);
+        return;
+    }
+    const cpx<T> *tw = lds_twiddles(s1 + Pz * in.tyz, twb + in.twz, Pz);
     const int nyt = min(in.tyz, Ly - y0);
     for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
         const int zz = e / nyt, yy = e - zz * nyt;
         s[e] = cconj(src[(long)zz * Ly + y0 + yy]);
     }
     __syncthreads();
-    s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, twb + in.twz);
+    s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, tw);
     for (int e = threadIdx.x; e < Lz * nyt; e += kThreads) {
         const int zz = e / nyt, yy = e - zz * nyt;
         dst[(long)zz * Ly + y0 + yy] = cconj(s[e]);
@@ -383,6 +444,7 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
     const Inst &in = I[w.inst];
     const int kx0 = w.a, kz = w.b, ncol = w.c;
     cpx<T> *s1 = s + 3 * in.ncol * in.Py;
+    const cpx<T> *tw = lds_twiddles(s1 + 3 * in.ncol * in.Py, twb + in.twy, in.Py);
     const int H = in.Hx, Pz = in.Pz, Py = in.Py, Ly = in.ny;
     cpx<T> *buf = bufb + in.b_off;
     const cpx<T> *kh = khatb + in.khat_off;
@@ -397,7 +459,7 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
         s[e] = v;
     }
     __syncthreads();
-    s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, twb + in.twy);
+    s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, tw);
     s1 = (s == s1) ? reinterpret_cast<cpx<T> *>(g_smem) : s1;
     for (int e = threadIdx.x; e < ncol * Py; e += kThreads) {
         const int col = e / Py, ky = e - col * Py;
@@ -417,7 +479,7 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
         *pz = cconj(hz);
     }
     __syncthreads();
-    s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, twb + in.twy);
+    s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, tw);
     for (int e = threadIdx.x; e < rows * Ly; e += kThreads) {
         const int r = e / Ly, y = e - r * Ly;
         const int comp = r / ncol, col = r - comp * ncol;
@@ -441,9 +503,10 @@ __global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict
     const long off = ((long)comp * H + kx) * Pz * Py + (long)kz * Py;
     const cpx<double> *src = kab + in.ka_off + off;
     cpx<TO> *dst = khatb + in.khat_off + off;
+    const cpx<double> *tw = lds_twiddles(s + 2 * Py, twb + in.twy, Py);
     for (int e = threadIdx.x; e < Py; e += kThreads) s[e] = src[e];
     __syncthreads();
-    s = fft_rows<double>(s, s + Py, 1, 1, Py, in.ply, twb + in.twy);
+    s = fft_rows<double>(s, s + Py, 1, 1, Py, in.ply, tw);
     const double scale = 1.0 / (double)in.NP;
     for (int e = threadIdx.x; e < Py; e += kThreads)
         dst[e] = {TO(s[e].x * scale), TO(s[e].y * scale)};
@@ -777,13 +840,29 @@ __global__ void k_set_ctl(uint32_t *ctl, uint32_t stage, uint32_t sweep_base)
 // ============================================================================
 // launch wrappers
 // ============================================================================
-static inline hipError_t post() { return hipGetLastError(); }
+static thread_local hipEvent_t g_ev_a = nullptr, g_ev_b = nullptr;
+
+void set_launch_events(hipEvent_t a, hipEvent_t b)
+{
+    g_ev_a = a;
+    g_ev_b = b;
+}
+
+static inline hipError_t post()
+{
+    g_ev_a = g_ev_b = nullptr;
+    return hipGetLastError();
+}
+
+// hipExtLaunchKernelGGL: with null events it is a plain launch
+#define MMC_LAUNCH(kern, grid, block, lds, st, ...) \
+    hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)(lds), st, g_ev_a, g_ev_b, 0u, __VA_ARGS__)
 
 hipError_t launch_tensor(hipStream_t st, const Inst *I, const Work *W, int n, double *K,
                          int ix_lo, int ix_hi, int first, int last)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tensor, dim3(n), dim3(kThreads), 0, st, I, W, K, ix_lo, ix_hi, first,
+    MMC_LAUNCH(k_tensor, dim3(n), dim3(kThreads), 0, st, I, W, K, ix_lo, ix_hi, first,
                        last);
     return post();
 }
@@ -793,7 +872,7 @@ hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int 
                        const T *real, cpx<T> *spec, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, real, spec, tw);
+    MMC_LAUNCH(k_xfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, real, spec, tw);
     return post();
 }
 
@@ -802,7 +881,7 @@ hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n, int 
                        const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, src, dst, tw);
+    MMC_LAUNCH(k_zfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, src, dst, tw);
     return post();
 }
 
@@ -811,7 +890,7 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, cons
                        cpx<T> *dst, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, src, dst, tw);
+    MMC_LAUNCH(k_zinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, src, dst, tw);
     return post();
 }
 
@@ -820,7 +899,7 @@ hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, cpx<
                        const cpx<T> *khat, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ymul<T>, dim3(n), dim3(kThreads), lds, st, I, W, buf, khat, tw);
+    MMC_LAUNCH(k_ymul<T>, dim3(n), dim3(kThreads), lds, st, I, W, buf, khat, tw);
     return post();
 }
 
@@ -830,7 +909,7 @@ hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W, int 
                               size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_yfwd_tensor<TO>, dim3(n), dim3(kThreads), lds, st, I, W, ka, khat, tw);
+    MMC_LAUNCH(k_yfwd_tensor<TO>, dim3(n), dim3(kThreads), lds, st, I, W, ka, khat, tw);
     return post();
 }
 
@@ -839,7 +918,7 @@ hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
                        const cpx<T> *spec, T *real, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, spec, real, tw);
+    MMC_LAUNCH(k_xinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, spec, real, tw);
     return post();
 }
 
@@ -849,7 +928,7 @@ hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n, T *
                         int use_demag)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_metro<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, beta, ctl,
+    MMC_LAUNCH(k_metro<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, beta, ctl,
                        sweep_off, color, use_demag);
     return post();
 }
@@ -859,7 +938,7 @@ hipError_t launch_energy(hipStream_t st, const Inst *I, const Work *W, int n, co
                          const T *h, double *partials, int use_demag)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_energy<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, partials,
+    MMC_LAUNCH(k_energy<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, partials,
                        use_demag);
     return post();
 }
@@ -868,7 +947,7 @@ hipError_t launch_energy_final(hipStream_t st, const int *first, int ninst,
                                const double *partials, double *out)
 {
     if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_energy_final, dim3(ninst), dim3(kThreads), 0, st, first, partials, out);
+    MMC_LAUNCH(k_energy_final, dim3(ninst), dim3(kThreads), 0, st, first, partials, out);
     return post();
 }
 
@@ -877,7 +956,7 @@ hipError_t launch_field(hipStream_t st, const Inst *I, const Work *W, int n, con
                         const T *h, int terms, double *H)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_field<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, terms, H);
+    MMC_LAUNCH(k_field<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, terms, H);
     return post();
 }
 
@@ -885,13 +964,13 @@ template <typename T>
 hipError_t launch_init(hipStream_t st, const Inst *I, const Work *W, int n, T *m)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init<T>, dim3(n), dim3(kThreads), 0, st, I, W, m);
+    MMC_LAUNCH(k_init<T>, dim3(n), dim3(kThreads), 0, st, I, W, m);
     return post();
 }
 
 hipError_t launch_set_ctl(hipStream_t st, uint32_t *ctl, uint32_t stage, uint32_t sweep_base)
 {
-    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, ctl, stage, sweep_base);
+    MMC_LAUNCH(k_set_ctl, dim3(1), dim3(64), 0, st, ctl, stage, sweep_base);
     return post();
 }
 
