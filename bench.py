@@ -93,17 +93,17 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
     ksteps = ksteps or min(steps, 400)
     c.set_option("timing", 1)
     for k in mod.KINDS:
-        c.kernel_time(k, reset=True)
+        c.kernel_stats(k, reset=True)
     t1 = time.perf_counter()
     c.sweeps(0, warmup + steps, ksteps)
     c.sync()
     dt_eager = time.perf_counter() - t1
     kern = {}
     for k in mod.KINDS:
-        n, ms = c.kernel_time(k, reset=True)
+        n, ms, by = c.kernel_stats(k, reset=True)
         if n:
             kern[k] = dict(launches=n, total_ms=ms, avg_us=1e3 * ms / n,
-                           bytes_per_launch=c.kernel_bytes(k))
+                           bytes_per_launch=by / n, total_bytes=by)
     c.set_option("timing", 0)
     E = c.energy()
     return dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
@@ -129,9 +129,11 @@ def batch_params(count=64, master_seed=2024):
 
 
 def roofline(kern):
+    """Dominant kernel (largest device time) and its achieved bandwidth:
+    algorithmic bytes of its timed launches over their summed duration."""
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     k = kern[dom]
-    gbs = k["bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9
+    gbs = k["total_bytes"] / (k["total_ms"] * 1e-3) / 1e9
     return dom, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                      frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
                      kernel=dom, avg_us=round(k["avg_us"], 3),

@@ -191,14 +191,18 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * Unknown keys return MMC_EINVAL. */
 int mmc_set_option(mmc_ctx *ctx, const char *key, long value);
 
-/* Launch count and summed device time [ms] of one kernel kind recorded
- * while "timing" was on ("xfwd", "zfwd", "ymul", "zinv", "xinv", "metro",
- * "energy", "tensor").  Resets the counters when reset != 0. */
-int mmc_kernel_time(mmc_ctx *ctx, const char *kind, long *launches,
-                    double *total_ms, int reset);
+/* Launches, summed device time [ms] and summed algorithmic HBM bytes of
+ * one kernel kind ("xfwd", "zfwd", "ymul", "zinv", "xinv", "metro",
+ * "energy", "tensor") recorded while "timing" was on; total_bytes /
+ * total_ms is the kind's achieved bandwidth.  Resets the counters when
+ * reset != 0.  Instrumentation only; no reference counterpart (the
+ * reference prints a CPU-clock "stages/s", src/main.cu:795-827). */
+int mmc_kernel_stats(mmc_ctx *ctx, const char *kind, long *launches,
+                     double *total_ms, double *total_bytes, int reset);
 
-/* Algorithmic HBM bytes of one launch of `kind` summed over the
- * instances (the roofline numerator used by the benchmark). */
+/* Algorithmic HBM bytes of one pass of `kind` over all instances (each
+ * array the pass touches read or written once; a pass may take several
+ * launches, one per kernel variant). */
 int mmc_kernel_bytes(const mmc_ctx *ctx, const char *kind, double *bytes);
 
 #ifdef __cplusplus

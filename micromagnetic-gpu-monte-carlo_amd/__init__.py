@@ -109,7 +109,7 @@ EXPORTS = (
     "mmc_get_tensor", "mmc_mirror_tensor", "mmc_set_m", "mmc_get_m",
     "mmc_init_random_m", "mmc_field", "mmc_energy", "mmc_set_temperature",
     "mmc_sweeps", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
-    "mmc_kernel_time", "mmc_kernel_bytes",
+    "mmc_kernel_stats", "mmc_kernel_bytes",
     "mmc_ovf_read", "mmc_ovf_write", "mmc_ovf_mesh_equal", "mmc_ini_parse",
 )
 
@@ -173,8 +173,8 @@ def lib() -> C.CDLL:
     L.mmc_anneal.argtypes = [vp, C.POINTER(Schedule), ENERGY_CB, SNAPSHOT_CB,
                              vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
     L.mmc_set_option.argtypes = [vp, C.c_char_p, C.c_long]
-    L.mmc_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_long), dp,
-                                  C.c_int]
+    L.mmc_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_long), dp, dp,
+                                   C.c_int]
     L.mmc_kernel_bytes.argtypes = [vp, C.c_char_p, dp]
     L.mmc_ovf_read.argtypes = [C.c_char_p, C.POINTER(OvfInfo), dp, dp, dp, C.c_long,
                                C.c_char_p, C.c_int]
@@ -381,12 +381,14 @@ class Context:
     def set_option(self, key: str, value: int):
         self._check(lib().mmc_set_option(self._ptr, key.encode(), int(value)))
 
-    def kernel_time(self, kind: str, reset: bool = False):
+    def kernel_stats(self, kind: str, reset: bool = False):
+        """(launches, total_ms, total_bytes) recorded while timing was on."""
         n = C.c_long()
         ms = C.c_double()
-        self._check(lib().mmc_kernel_time(self._ptr, kind.encode(), C.byref(n),
-                                          C.byref(ms), int(reset)))
-        return n.value, ms.value
+        by = C.c_double()
+        self._check(lib().mmc_kernel_stats(self._ptr, kind.encode(), C.byref(n),
+                                           C.byref(ms), C.byref(by), int(reset)))
+        return n.value, ms.value, by.value
 
     def kernel_bytes(self, kind: str) -> float:
         b = C.c_double()

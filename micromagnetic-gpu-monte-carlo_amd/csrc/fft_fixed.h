@@ -1,0 +1,100 @@
+// fft_fixed.h -- Stockham FFT with a compile-time plan, in place in one LDS
+// buffer.
+//
+// For the y lengths that occur most (2^k and 3*2^k) the radix sequence,
+// butterfly counts and strides are template constants, so each stage is
+// straight-line code: every thread loads all of its butterflies of the stage
+// into registers, the workgroup synchronises, and the outputs go back to the
+// same buffer.  One buffer instead of two halves the LDS of the generic
+// ping-pong path (fft_lds.h), so more workgroups stay resident per CU.  The
+// arithmetic per butterfly (twiddle product, R-point DFT) is the generic
+// path's, so both paths give bit-identical transforms.
+#pragma once
+
+#include "fft_lds.h"
+
+namespace mmc {
+
+template <int... Rs> struct Radices {};
+
+template <int N> struct FixedPlan;   // only the sizes listed are specialised
+template <> struct FixedPlan<64> { using type = Radices<4, 4, 4>; };
+template <> struct FixedPlan<96> { using type = Radices<4, 4, 2, 3>; };
+template <> struct FixedPlan<128> { using type = Radices<4, 4, 4, 2>; };
+template <> struct FixedPlan<192> { using type = Radices<4, 4, 4, 3>; };
+template <> struct FixedPlan<256> { using type = Radices<4, 4, 4, 4>; };
+template <> struct FixedPlan<384> { using type = Radices<4, 4, 4, 2, 3>; };
+template <> struct FixedPlan<512> { using type = Radices<4, 4, 4, 4, 2>; };
+template <> struct FixedPlan<768> { using type = Radices<4, 4, 4, 4, 3>; };
+template <> struct FixedPlan<1024> { using type = Radices<4, 4, 4, 4, 4>; };
+template <> struct FixedPlan<2048> { using type = Radices<4, 4, 4, 4, 4, 2>; };
+
+// Same radix order as the host planner (make_plan: 4s first, then 2, 3, 5, 7).
+__host__ __device__ constexpr bool fixed_fft_size(int n)
+{
+    return n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 384 ||
+           n == 512 || n == 768 || n == 1024 || n == 2048;
+}
+
+template <typename T, int N, int ROWS, int NS, int R>
+__device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict__ tw)
+{
+    constexpr int NB = N / R;
+    constexpr int TOTAL = NB * ROWS;
+    constexpr int PER = (TOTAL + kThreads - 1) / kThreads;
+    constexpr int TWS = N / (NS * R);
+    cpx<T> v[PER][R];
+    #pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int t = threadIdx.x + b * kThreads;
+        if ((b + 1) * kThreads <= TOTAL || t < TOTAL) {
+            const int row = t / NB, j = t % NB;
+            #pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = s[row * N + j + r * NB];
+        }
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int t = threadIdx.x + b * kThreads;
+        if ((b + 1) * kThreads <= TOTAL || t < TOTAL) {
+            const int row = t / NB, j = t % NB;
+            const int k = j % NS;
+            if (NS > 1) {
+                #pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * TWS]);
+            }
+            Dft<T, R>::run(v[b]);
+            const int o = (j - k) * R + k;
+            #pragma unroll
+            for (int r = 0; r < R; ++r) s[row * N + o + r * NS] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int ROWS, int NS, typename RL> struct FixedStages;
+template <typename T, int N, int ROWS, int NS> struct FixedStages<T, N, ROWS, NS, Radices<>> {
+    static __device__ __forceinline__ void run(cpx<T> *, const cpx<T> *) {}
+};
+template <typename T, int N, int ROWS, int NS, int R, int... Rest>
+struct FixedStages<T, N, ROWS, NS, Radices<R, Rest...>> {
+    static __device__ __forceinline__ void run(cpx<T> *s, const cpx<T> *tw)
+    {
+        fixed_stage<T, N, ROWS, NS, R>(s, tw);
+        FixedStages<T, N, ROWS, NS * R, Radices<Rest...>>::run(s, tw);
+    }
+};
+
+// Forward transform of ROWS contiguous rows of length N held in s (row
+// stride N).  Caller synchronises after writing s; synchronised on return.
+template <typename T, int N, int ROWS>
+__device__ __forceinline__ void fft_fixed(cpx<T> *s, const cpx<T> *__restrict__ tw)
+{
+    FixedStages<T, N, ROWS, 1, typename FixedPlan<N>::type>::run(s, tw);
+}
+
+// Columns per y-pass workgroup of the fixed path: about 3072 complex per tile.
+__host__ __device__ constexpr int fixed_cols(int n) { return n >= 1024 ? 1 : 1024 / n; }
+
+}  // namespace mmc

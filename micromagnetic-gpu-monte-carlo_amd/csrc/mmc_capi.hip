@@ -6,6 +6,7 @@
 // HIP stream per context.
 #include "mmc.h"
 #include "mmc_internal.h"
+#include "fft_fixed.h"
 
 #include <algorithm>
 #include <cmath>
@@ -29,9 +30,18 @@ const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro
 constexpr int kNumKinds = 8;
 enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY };
 
+struct Segment {
+    int fixed_n = 0;   // common FFT length of the segment's kernel, 0 = generic
+    int rows = 0;      // x pass: row pairs per fixed-plan tile
+    int begin = 0, count = 0;
+    size_t lds = 0;
+    double bytes = 0;  // algorithmic HBM bytes of one launch of the segment
+};
+
 struct List {
     std::vector<Work> host;
     std::vector<int> first;   // first item of each instance, size count+1
+    std::vector<Segment> seg; // x and y passes: items grouped by kernel variant
     Work *dev = nullptr;
     size_t lds = 0;
     int n() const { return (int)host.size(); }
@@ -65,6 +75,14 @@ bool make_plan(int n, FftPlan &pl)
         }
     return v == 1;
 }
+
+// x lengths with fixed-plan x-pass kernels, and their full tile (row pairs)
+bool xfixed_size(int n)
+{
+    return n == 128 || n == 192 || n == 256 || n == 384 || n == 512 || n == 1024 || n == 2048;
+}
+
+int xrows_for(size_t tsize) { return tsize == 4 ? kXRows<float>::value : kXRows<double>::value; }
 
 }  // namespace
 
@@ -106,12 +124,13 @@ struct mmc_ctx {
     hipGraph_t graph = nullptr;
     int graph_sweeps = 0;
 
-    struct Rec { int kind; hipEvent_t a, b; };
+    struct Rec { int kind; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     long launches[kNumKinds] = {0};
     double total_ms[kNumKinds] = {0};
+    double total_bytes[kNumKinds] = {0};
 
     std::string err;
 };
@@ -156,14 +175,18 @@ void harvest(mmc_ctx *c)
         if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
             c->launches[r.kind]++;
             c->total_ms[r.kind] += ms;
+            c->total_bytes[r.kind] += r.bytes;
         }
     }
     c->recs.clear();
     c->ev_used = 0;
 }
 
+// Bytes of a launch that covers every instance (unsegmented lists).
+double all_bytes(const mmc_ctx *c, int kind);
+
 template <typename F>
-hipError_t timed(mmc_ctx *c, int kind, F &&f)
+hipError_t timed(mmc_ctx *c, int kind, F &&f, double bytes = -1.0)
 {
     if (!c->timing) return f();
     if (c->ev_used + 2 > 4096) harvest(c);
@@ -172,7 +195,7 @@ hipError_t timed(mmc_ctx *c, int kind, F &&f)
     set_launch_events(a, b);
     hipError_t e = f();
     set_launch_events(nullptr, nullptr);
-    c->recs.push_back({kind, a, b});
+    c->recs.push_back({kind, a, b, bytes >= 0 ? bytes : all_bytes(c, kind)});
     return e;
 }
 
@@ -199,10 +222,14 @@ bool choose_tiles(const mmc_ctx *c, Inst &in, size_t pref, size_t maxb)
 {
     const size_t cs = 2 * c->tsize;
     const int tymax = c->tsize == 4 ? 16 : 8;
-    const int pairs = fit_tw(tymax / 2, 2 * (size_t)in.Px * cs, in.Px * cs, pref, maxb);
+    int pairs = fit_tw(tymax / 2, 2 * (size_t)in.Px * cs, in.Px * cs, pref, maxb);
+    if (xfixed_size(in.Px) && (size_t)(xrows_for(c->tsize) + 1) * in.Px * cs <= maxb)
+        pairs = xrows_for(c->tsize);
     in.tyx = 2 * pairs;
     in.tyz = in.Pz <= kZDirect ? 1 : fit_tw(tymax, 2 * (size_t)in.Pz * cs, in.Pz * cs, pref, maxb);
-    in.ncol = fit_tw(4, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, pref, maxb);
+    in.ncol = fixed_fft_size(in.Py) ? fixed_cols(in.Py)
+                                    : fit_tw(4, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, pref, maxb);
+    if (fixed_fft_size(in.Py) && (3 * (size_t)fixed_cols(in.Py) + 1) * in.Py * cs > maxb) in.ncol = 0;
     in.tyx64 = 2 * fit_tw(4, 2 * (size_t)in.Px * 16, in.Px * 16, pref, maxb);
     in.tyz64 = in.Pz <= kZDirect ? 1 : fit_tw(8, 2 * (size_t)in.Pz * 16, in.Pz * 16, pref, maxb);
     const bool yt = 3 * (size_t)in.Py * 16 <= maxb;
@@ -274,10 +301,54 @@ int upload_list(mmc_ctx *c, List &l)
     return MMC_OK;
 }
 
+// Algorithmic HBM bytes of one pass of kernel `k` over instance `in`
+// (b = element size): each array read or written once.
+double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
+{
+    const double b = (double)c->tsize, cb = 2.0 * b;
+    const double N = (double)in.N;
+    const double spec_x = 3.0 * in.Hx * in.nz * in.ny * cb;   // A
+    const double spec_z = 3.0 * in.Hx * in.Pz * in.ny * cb;   // B
+    const double khat = 6.0 * in.Hx * in.Pz * in.Py * cb;
+    switch (k) {
+    case K_XFWD: case K_XINV: return in.cal_demag ? 3 * N * b + spec_x : 0.0;
+    case K_ZFWD: case K_ZINV: return (in.cal_demag && in.Pz > 1) ? spec_x + spec_z : 0.0;
+    case K_YMUL: return in.cal_demag ? 2 * spec_z + khat : 0.0;
+    case K_METRO: {
+        // one colour pass: all of m read (cell + neighbours), h of the
+        // colour's cells and their updated m
+        const double K = in.ncolors;
+        return 3 * N * b + (in.cal_demag ? 3 * N * b / K : 0.0) + 3 * N * b / K;
+    }
+    case K_ENERGY: return 3 * N * b + (in.cal_demag ? 3 * N * b : 0.0);
+    case K_TENSOR: return 6.0 * in.NP * 8;
+    }
+    return 0.0;
+}
+
+double all_bytes(const mmc_ctx *c, int kind)
+{
+    double t = 0;
+    for (auto &in : c->inst) t += inst_bytes(c, in, kind);
+    return t;
+}
+
+
+// Does instance `in` take the fixed-plan x kernels (full tile fits LDS)?
+bool xfixed(const mmc_ctx *c, const Inst &in)
+{
+    return xfixed_size(in.Px) &&
+           (size_t)(xrows_for(c->tsize) + 1) * in.Px * 2 * c->tsize <= c->lds_max;
+}
+
 void build_lists(mmc_ctx *c)
 {
     const size_t cs = 2 * c->tsize;   // complex element size
-    auto reset = [&](List &l) { l.host.clear(); l.first.assign(1, 0); l.lds = 0; };
+    auto reset = [&](List &l) { l.host.clear(); l.first.assign(1, 0); l.lds = 0; l.seg.clear(); };
+    std::map<int, Segment> yclass;           // fixed length (0: generic) -> items
+    std::map<int, std::vector<Work>> yitems;
+    std::map<std::pair<int, int>, Segment> xclass;   // (fixed length, rows) -> items
+    std::map<std::pair<int, int>, std::vector<Work>> xitems;
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
@@ -285,20 +356,36 @@ void build_lists(mmc_ctx *c)
         const int zstep = zdirect ? kThreads : in.tyz;
         const int zstep64 = zdirect ? kThreads : in.tyz64;
         if (in.cal_demag) {
+            const bool xf = xfixed(c, in);
+            const std::pair<int, int> xk = xf ? std::make_pair(in.Px, in.tyx / 2) : std::make_pair(0, 0);
             for (int comp = 0; comp < 3; ++comp)
                 for (int z = 0; z < in.nz; ++z)
-                    for (int y0 = 0; y0 < in.ny; y0 += in.tyx) c->xl.host.push_back({i, comp, z, y0});
-            c->xl.lds = std::max(c->xl.lds, (2 * (size_t)(in.tyx / 2) + 1) * in.Px * cs);
+                    for (int y0 = 0; y0 < in.ny; y0 += in.tyx) xitems[xk].push_back({i, comp, z, y0});
+            const size_t xlds = xf ? ((size_t)in.tyx / 2 + 1) * in.Px * cs
+                                   : (2 * (size_t)(in.tyx / 2) + 1) * in.Px * cs;
+            Segment &xs = xclass[xk];
+            xs.fixed_n = xk.first;
+            xs.rows = xk.second;
+            xs.lds = std::max(xs.lds, xlds);
+            xs.bytes += inst_bytes(c, in, K_XFWD);
+            c->xl.lds = std::max(c->xl.lds, xlds);
             if (in.Pz > 1) {
                 for (int comp = 0; comp < 3; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
                         for (int y0 = 0; y0 < in.ny; y0 += zstep) c->zl.host.push_back({i, comp, kx, y0});
                 if (!zdirect) c->zl.lds = std::max(c->zl.lds, (2 * (size_t)in.tyz + 1) * in.Pz * cs);
             }
+            const int fn = fixed_fft_size(in.Py) ? in.Py : 0;
             for (int kz = 0; kz < in.Pz; ++kz)
                 for (int kx0 = 0; kx0 < in.Hx; kx0 += in.ncol)
-                    c->yl.host.push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
-            c->yl.lds = std::max(c->yl.lds, (2 * (size_t)3 * in.ncol + 1) * in.Py * cs);
+                    yitems[fn].push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
+            const size_t ylds = fn ? (3 * (size_t)fixed_cols(in.Py) + 1) * in.Py * cs
+                                   : (2 * (size_t)3 * in.ncol + 1) * in.Py * cs;
+            Segment &sg = yclass[fn];
+            sg.fixed_n = fn;
+            sg.lds = std::max(sg.lds, ylds);
+            sg.bytes += inst_bytes(c, in, K_YMUL);
+            c->yl.lds = std::max(c->yl.lds, ylds);
             // tensor passes (FP64)
             for (int comp = 0; comp < 6; ++comp)
                 for (int z = 0; z < in.Pz; ++z)
@@ -323,6 +410,21 @@ void build_lists(mmc_ctx *c)
         for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl})
             l->first.push_back(l->n());
     }
+    // x and y passes: one contiguous segment (and launch) per kernel variant
+    for (auto &kv : xitems) {
+        Segment sg = xclass[kv.first];
+        sg.begin = c->xl.n();
+        sg.count = (int)kv.second.size();
+        c->xl.host.insert(c->xl.host.end(), kv.second.begin(), kv.second.end());
+        c->xl.seg.push_back(sg);
+    }
+    for (auto &kv : yitems) {
+        Segment sg = yclass[kv.first];
+        sg.begin = c->yl.n();
+        sg.count = (int)kv.second.size();
+        c->yl.host.insert(c->yl.host.end(), kv.second.begin(), kv.second.end());
+        c->yl.seg.push_back(sg);
+    }
 }
 
 template <typename T>
@@ -344,6 +446,7 @@ int max_tyx(const mmc_ctx *c, const Inst &in)
 
 int max_ncol(const mmc_ctx *c, const Inst &in)
 {
+    if (fixed_fft_size(in.Py)) return fixed_cols(in.Py);
     const size_t cs = 2 * c->tsize;
     return fit_tw(64, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, c->lds_max, c->lds_max);
 }
@@ -369,7 +472,11 @@ void adapt_tiles(mmc_ctx *c)
     for (bool changed = true; changed && xitems() < target;) {
         changed = false;
         for (auto &in : c->inst)
-            if (in.tyx > 2) { in.tyx = std::max(2, (in.tyx / 2) & ~1); changed = true; }
+            if (in.tyx > 2) {
+                // fixed-plan x tiles come in two shapes: full, or one row pair
+                in.tyx = xfixed(c, in) ? 2 : std::max(2, (in.tyx / 2) & ~1);
+                changed = true;
+            }
     }
     for (bool changed = true; changed && yitems() < target;) {
         changed = false;
@@ -502,7 +609,10 @@ int retile(mmc_ctx *c, int tyx, int ncol)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (auto &in : c->inst) {
-        if (tyx > 0) in.tyx = std::max(2, std::min(tyx & ~1, max_tyx(c, in)));
+        if (tyx > 0) {
+            in.tyx = std::max(2, std::min(tyx & ~1, max_tyx(c, in)));
+            if (xfixed(c, in)) in.tyx = in.tyx >= 2 * xrows_for(c->tsize) ? 2 * xrows_for(c->tsize) : 2;
+        }
         if (ncol > 0) in.ncol = std::max(1, std::min(ncol, max_ncol(c, in)));
     }
     build_lists(c);
@@ -529,13 +639,25 @@ int demag_chain(mmc_ctx *c)
     cpx<T> *A = (cpx<T> *)c->d_a;
     cpx<T> *B = (cpx<T> *)c->d_b;
     const cpx<T> *Kh = (const cpx<T> *)c->d_khat;
-    HIPCHK(c, timed(c, K_XFWD, [&] { return launch_xfwd<T>(st, c->dInst, c->xl.dev, c->xl.n(), 0, m, A, tw, c->xl.lds); }));
+    for (const Segment &sg : c->xl.seg)
+        HIPCHK(c, timed(c, K_XFWD, [&] {
+                   return launch_xfwd<T>(st, c->dInst, c->xl.dev + sg.begin, sg.count, 0, sg.fixed_n,
+                                         sg.rows, m, A, tw, sg.lds);
+               }, sg.bytes));
     if (c->zl.n())
         HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(st, c->dInst, c->zl.dev, c->zl.n(), 0, A, B, tw, c->zl.lds); }));
-    HIPCHK(c, timed(c, K_YMUL, [&] { return launch_ymul<T>(st, c->dInst, c->yl.dev, c->yl.n(), B, Kh, tw, c->yl.lds); }));
+    for (const Segment &sg : c->yl.seg)
+        HIPCHK(c, timed(c, K_YMUL, [&] {
+                   return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, B, Kh,
+                                         tw, sg.lds);
+               }, sg.bytes));
     if (c->zl.n())
         HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, c->zl.dev, c->zl.n(), B, A, tw, c->zl.lds); }));
-    HIPCHK(c, timed(c, K_XINV, [&] { return launch_xinv<T>(st, c->dInst, c->xl.dev, c->xl.n(), A, h, tw, c->xl.lds); }));
+    for (const Segment &sg : c->xl.seg)
+        HIPCHK(c, timed(c, K_XINV, [&] {
+                   return launch_xinv<T>(st, c->dInst, c->xl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
+                                         A, h, tw, sg.lds);
+               }, sg.bytes));
     return MMC_OK;
 }
 
@@ -676,7 +798,8 @@ int tensor_fft(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&KA, std::max<size_t>(1, c->ka_elems) * sizeof(cpx<double>)));
     const cpx<double> *tw = (const cpx<double> *)c->d_tw64;
     int rc = MMC_OK;
-    hipError_t e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, c->d_K, KA, tw, c->txl.lds);
+    hipError_t e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, 0, 0, c->d_K, KA, tw,
+                                       c->txl.lds);
     if (e == hipSuccess && c->tzl.n())
         e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, KA, KA, tw, c->tzl.lds);
     if (e == hipSuccess)
@@ -1102,7 +1225,8 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
     return fail(c, MMC_EINVAL, "unknown option '%s'", key);
 }
 
-int mmc_kernel_time(mmc_ctx *c, const char *kind, long *launches, double *total_ms, int reset)
+int mmc_kernel_stats(mmc_ctx *c, const char *kind, long *launches, double *total_ms,
+                     double *total_bytes, int reset)
 {
     if (!c || !kind) return MMC_EINVAL;
     harvest(c);
@@ -1110,7 +1234,8 @@ int mmc_kernel_time(mmc_ctx *c, const char *kind, long *launches, double *total_
         if (!strcmp(kind, kKinds[k])) {
             if (launches) *launches = c->launches[k];
             if (total_ms) *total_ms = c->total_ms[k];
-            if (reset) { c->launches[k] = 0; c->total_ms[k] = 0; }
+            if (total_bytes) *total_bytes = c->total_bytes[k];
+            if (reset) { c->launches[k] = 0; c->total_ms[k] = 0; c->total_bytes[k] = 0; }
             return MMC_OK;
         }
     return fail(c, MMC_EINVAL, "unknown kernel kind '%s'", kind);
@@ -1120,28 +1245,12 @@ int mmc_kernel_bytes(const mmc_ctx *cc, const char *kind, double *bytes)
 {
     mmc_ctx *c = const_cast<mmc_ctx *>(cc);
     if (!c || !kind || !bytes) return MMC_EINVAL;
-    const double b = (double)c->tsize, cb = 2.0 * b;
-    double tot = 0;
-    std::string k(kind);
-    for (auto &in : c->inst) {
-        const double N = (double)in.N;
-        const double spec_x = 3.0 * in.Hx * in.nz * in.ny * cb;   // A
-        const double spec_z = 3.0 * in.Hx * in.Pz * in.ny * cb;   // B
-        const double khat = 6.0 * in.Hx * in.Pz * in.Py * cb;
-        if (k == "xfwd" || k == "xinv") { if (in.cal_demag) tot += 3 * N * b + spec_x; }
-        else if (k == "zfwd" || k == "zinv") { if (in.cal_demag && in.Pz > 1) tot += spec_x + spec_z; }
-        else if (k == "ymul") { if (in.cal_demag) tot += 2 * spec_z + khat; }
-        else if (k == "metro") {
-            // one colour pass: all of m read (cell + neighbours), h and the
-            // updated m of this colour's cells
-            const double K = in.ncolors;
-            tot += 3 * N * b + (in.cal_demag ? 3 * N * b / K : 0.0) + 3 * N * b / K;
-        } else if (k == "energy") tot += 3 * N * b + (in.cal_demag ? 3 * N * b : 0.0);
-        else if (k == "tensor") tot += 6.0 * in.NP * 8;
-        else return fail(c, MMC_EINVAL, "unknown kernel kind '%s'", kind);
-    }
-    *bytes = tot;
-    return MMC_OK;
+    for (int k = 0; k < kNumKinds; ++k)
+        if (!strcmp(kind, kKinds[k])) {
+            *bytes = all_bytes(c, k);
+            return MMC_OK;
+        }
+    return fail(c, MMC_EINVAL, "unknown kernel kind '%s'", kind);
 }
 
 }  // extern "C"

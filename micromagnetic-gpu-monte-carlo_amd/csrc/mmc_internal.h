@@ -76,10 +76,18 @@ struct Work {
 hipError_t launch_tensor(hipStream_t st, const Inst *I, const Work *W, int n,
                          double *K, int ix_lo, int ix_hi, int first, int last);
 
+// Row pairs of a full fixed-plan x-pass tile (16 FP32 / 8 FP64 real rows,
+// so a tile's stores along y fill 128-byte segments).
+template <typename T> struct kXRows;
+template <> struct kXRows<float> { static constexpr int value = 8; };
+template <> struct kXRows<double> { static constexpr int value = 4; };
+
+// fixed_n, rows: x length and row pairs of a fixed-plan tile (magnetisation
+// mode only), or fixed_n = 0 for the generic kernel.
 template <typename T>
 hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n,
-                       int mode, const T *real, cpx<T> *spec,
-                       const cpx<T> *tw, size_t lds);
+                       int mode, int fixed_n, int rows, const T *real,
+                       cpx<T> *spec, const cpx<T> *tw, size_t lds);
 template <typename T>
 hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n,
                        int mode, const cpx<T> *src, cpx<T> *dst,
@@ -88,18 +96,20 @@ template <typename T>
 hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n,
                        const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw,
                        size_t lds);
+// fixed_n: the common y length of all items (compile-time-planned kernel),
+// or 0 for the generic kernel.
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
-                       cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw,
-                       size_t lds);
+                       int fixed_n, cpx<T> *buf, const cpx<T> *khat,
+                       const cpx<T> *tw, size_t lds);
 template <typename TO>
 hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W,
                               int n, const cpx<double> *ka, cpx<TO> *khat,
                               const cpx<double> *tw, size_t lds);
 template <typename T>
 hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
-                       const cpx<T> *spec, T *real, const cpx<T> *tw,
-                       size_t lds);
+                       int fixed_n, int rows, const cpx<T> *spec, T *real,
+                       const cpx<T> *tw, size_t lds);
 template <typename T>
 hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n,
                         T *m, const T *h, const double *beta,

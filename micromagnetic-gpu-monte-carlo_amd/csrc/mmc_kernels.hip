@@ -17,6 +17,7 @@
 //   k_field         effective field in A/m (effectiveHKernel,
 //                   src/effective_H.h:25-33)
 //   k_init          Philox uniform-on-sphere start state
+#include "fft_fixed.h"
 #include "fft_lds.h"
 #include "mmc_internal.h"
 
@@ -316,6 +317,87 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
     }
 }
 
+// Fixed-plan x passes (magnetisation mode only): Px == N for every item, a
+// tile of 2*ROWS real rows packed into ROWS complex rows, transformed in
+// place in one LDS buffer (rows past the tile's last pair are zero).
+template <typename T, int N, int ROWS>
+__global__ __launch_bounds__(kThreads) void k_xfwd_n(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     const T *__restrict__ realb,
+                                                     cpx<T> *__restrict__ specb,
+                                                     const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + ROWS * N;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, z = w.b, y0 = w.c;
+    constexpr int H = N / 2 + 1;
+    const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
+    const T *src = realb + in.m_off + (long)c * in.N + (long)z * Lx * Ly;
+    cpx<T> *dst = specb + in.a_off + (long)c * H * Lz * Ly;
+    const int nyt = min(2 * ROWS, Ly - y0);
+    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
+    for (int e = threadIdx.x; e < ROWS * N; e += kThreads) {
+        const int q = e / N, x = e % N;
+        T re = T(0), im = T(0);
+        if (x < Lx && 2 * q < nyt) {
+            const int ya = y0 + 2 * q;
+            re = src[(long)ya * Lx + x];
+            if (2 * q + 1 < nyt) im = src[(long)(ya + 1) * Lx + x];
+        }
+        s[e] = {re, im};
+    }
+    __syncthreads();
+    fft_fixed<T, N, ROWS>(s, tw);
+    const int npair = (nyt + 1) >> 1;
+    for (int e = threadIdx.x; e < H * npair; e += kThreads) {
+        const int k = e / npair, q = e - k * npair;
+        const cpx<T> A = s[q * N + k];
+        const cpx<T> B = s[q * N + (k ? N - k : 0)];
+        cpx<T> *o = dst + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+        o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+        if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+    }
+}
+
+template <typename T, int N, int ROWS>
+__global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     const cpx<T> *__restrict__ specb,
+                                                     T *__restrict__ realb,
+                                                     const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + ROWS * N;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, z = w.b, y0 = w.c;
+    constexpr int H = N / 2 + 1;
+    const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
+    const cpx<T> *src = specb + in.a_off + (long)c * H * Lz * Ly;
+    T *dst = realb + in.h_off + (long)c * in.N + (long)z * Lx * Ly;
+    const int nyt = min(2 * ROWS, Ly - y0);
+    const int npair = (nyt + 1) >> 1;
+    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
+    for (int e = threadIdx.x; e < (ROWS - npair) * N; e += kThreads) s[npair * N + e] = {T(0), T(0)};
+    for (int e = threadIdx.x; e < H * npair; e += kThreads) {
+        const int k = e / npair, q = e - k * npair;
+        const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+        const cpx<T> a = p[0];
+        const cpx<T> b = (2 * q + 1 < nyt) ? p[1] : cpx<T>{T(0), T(0)};
+        s[q * N + k] = {a.x - b.y, -(a.y + b.x)};
+        if (k > 0 && N - k >= H) s[q * N + N - k] = {a.x + b.y, a.y - b.x};
+    }
+    __syncthreads();
+    fft_fixed<T, N, ROWS>(s, tw);
+    for (int e = threadIdx.x; e < nyt * Lx; e += kThreads) {
+        const int yy = e / Lx, x = e - yy * Lx;
+        const cpx<T> v = s[(yy >> 1) * N + x];
+        dst[(long)(y0 + yy) * Lx + x] = (yy & 1) ? -v.y : v.x;
+    }
+}
+
 // Direct DFT along z of one y column (element z at p[z*stride]): nin inputs
 // (zero above), nout outputs, length P <= kZDirect, forward or inverse
 // (conjugate twiddles).  In place safe: all inputs are read first.
@@ -430,8 +512,95 @@ __global__ __launch_bounds__(kThreads) void k_zinv(const Inst *__restrict__ I,
     }
 }
 
+// Symmetric 3x3 complex product (src/demag_cu.cu:98-111) of one k point,
+// in place, conjugated for the inverse transform that follows.
+template <typename T>
+__device__ __forceinline__ void kmul_conj(cpx<T> *px, cpx<T> *py, cpx<T> *pz, const cpx<T> *k6)
+{
+    const cpx<T> mx = *px, my = *py, mz = *pz;
+    const cpx<T> hx = cadd(cadd(cmul(mx, k6[0]), cmul(my, k6[1])), cmul(mz, k6[2]));
+    const cpx<T> hy = cadd(cadd(cmul(mx, k6[1]), cmul(my, k6[3])), cmul(mz, k6[4]));
+    const cpx<T> hz = cadd(cadd(cmul(mx, k6[2]), cmul(my, k6[4])), cmul(mz, k6[5]));
+    *px = cconj(hx);
+    *py = cconj(hy);
+    *pz = cconj(hz);
+}
+
+// Compile-time-planned y pass for Py = N: one LDS buffer of 3*C rows
+// (row comp*C + col), the item's Khat values prefetched into registers
+// before the forward transform so their HBM latency overlaps it.
+template <typename T, int N>
+__device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T> *__restrict__ bufb,
+                                           const cpx<T> *__restrict__ khatb,
+                                           const cpx<T> *__restrict__ twb)
+{
+    constexpr int C = fixed_cols(N);
+    constexpr int ROWS = 3 * C;
+    constexpr int Q = (C * N + kThreads - 1) / kThreads;
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + ROWS * N;
+    const int kx0 = w.a, kz = w.b, ncol = w.c;
+    const int H = in.Hx, Pz = in.Pz, Ly = in.ny;
+    cpx<T> *buf = bufb + in.b_off;
+    const cpx<T> *kh = khatb + in.khat_off;
+    const long cstride = (long)H * Pz * Ly;
+    const long kstride = (long)H * Pz * N;
+    cpx<T> k6[Q][6];
+    #pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kThreads;
+        const int col = e / N, ky = e % N;
+        const bool ok = e < C * N && col < ncol;
+        const long ki = ((long)(kx0 + col) * Pz + kz) * N + ky;
+        #pragma unroll
+        for (int c6 = 0; c6 < 6; ++c6)
+            k6[q][c6] = ok ? kh[c6 * kstride + ki] : cpx<T>{T(0), T(0)};
+    }
+    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twy + e];
+    for (int e = threadIdx.x; e < ROWS * N; e += kThreads) {
+        const int r = e / N, y = e % N;
+        const int comp = r / C, col = r % C;
+        cpx<T> v = {T(0), T(0)};
+        if (col < ncol && y < Ly) v = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+        s[e] = v;
+    }
+    __syncthreads();
+    fft_fixed<T, N, ROWS>(s, tw);
+    #pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * kThreads;
+        if (e < C * N) {
+            const int col = e / N, ky = e % N;
+            kmul_conj(s + (0 * C + col) * N + ky, s + (1 * C + col) * N + ky,
+                      s + (2 * C + col) * N + ky, k6[q]);
+        }
+    }
+    __syncthreads();
+    fft_fixed<T, N, ROWS>(s, tw);
+    for (int e = threadIdx.x; e < ROWS * Ly; e += kThreads) {
+        const int r = e / Ly, y = e - r * Ly;
+        const int comp = r / C, col = r % C;
+        if (col < ncol)
+            buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y] = cconj(s[r * N + y]);
+    }
+}
+
+// Fixed-plan y pass: every item's instance has Py == N (one launch per
+// distinct length; a kernel of its own keeps its register count low).
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     cpx<T> *__restrict__ bufb,
+                                                     const cpx<T> *__restrict__ khatb,
+                                                     const cpx<T> *__restrict__ twb)
+{
+    const Work w = W[blockIdx.x];
+    ymul_fixed<T, N>(I[w.inst], w, bufb, khatb, twb);
+}
+
 // y forward, symmetric 3x3 product with Khat, y inverse.  Item (inst, kx0,
-// kz, ncol): columns kx0..kx0+ncol-1.  LDS row (comp, col) = comp*ncol+col.
+// kz, ncol): columns kx0..kx0+ncol-1; generic ping-pong path with LDS row
+// (comp, col) = comp*ncol+col.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
                                                    const Work *__restrict__ W,
@@ -439,9 +608,9 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
                                                    const cpx<T> *__restrict__ khatb,
                                                    const cpx<T> *__restrict__ twb)
 {
-    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     const int kx0 = w.a, kz = w.b, ncol = w.c;
     cpx<T> *s1 = s + 3 * in.ncol * in.Py;
     const cpx<T> *tw = lds_twiddles(s1 + 3 * in.ncol * in.Py, twb + in.twy, in.Py);
@@ -867,12 +1036,22 @@ hipError_t launch_tensor(hipStream_t st, const Inst *I, const Work *W, int n, do
     return post();
 }
 
+// Fixed x-pass variants: sizes x {1 row pair, kXRows<T> row pairs}.
+#define MMC_XFIXED(M, T, R) M(T, 128, R) M(T, 192, R) M(T, 256, R) M(T, 384, R) \
+    M(T, 512, R) M(T, 1024, R) M(T, 2048, R)
+
 template <typename T>
-hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode,
-                       const T *real, cpx<T> *spec, const cpx<T> *tw, size_t lds)
+hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode, int fixed_n,
+                       int rows, const T *real, cpx<T> *spec, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    MMC_LAUNCH(k_xfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, real, spec, tw);
+    const dim3 g(n), b(kThreads);
+    constexpr int RM = kXRows<T>::value;
+#define MMC_XF_CASE(TT, NN, RR) \
+    if (fixed_n == NN && rows == RR) { MMC_LAUNCH((k_xfwd_n<TT, NN, RR>), g, b, lds, st, I, W, real, spec, tw); return post(); }
+    if (mode == 0) { MMC_XFIXED(MMC_XF_CASE, T, 1) MMC_XFIXED(MMC_XF_CASE, T, RM) }
+#undef MMC_XF_CASE
+    MMC_LAUNCH(k_xfwd<T>, g, b, lds, st, I, W, mode, real, spec, tw);
     return post();
 }
 
@@ -895,11 +1074,24 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, cons
 }
 
 template <typename T>
-hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, cpx<T> *buf,
-                       const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
+                       cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    MMC_LAUNCH(k_ymul<T>, dim3(n), dim3(kThreads), lds, st, I, W, buf, khat, tw);
+    const dim3 g(n), b(kThreads);
+    switch (fixed_n) {
+    case 64: MMC_LAUNCH((k_ymul_n<T, 64>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 96: MMC_LAUNCH((k_ymul_n<T, 96>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 128: MMC_LAUNCH((k_ymul_n<T, 128>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 192: MMC_LAUNCH((k_ymul_n<T, 192>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 256: MMC_LAUNCH((k_ymul_n<T, 256>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 384: MMC_LAUNCH((k_ymul_n<T, 384>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 512: MMC_LAUNCH((k_ymul_n<T, 512>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 768: MMC_LAUNCH((k_ymul_n<T, 768>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 1024: MMC_LAUNCH((k_ymul_n<T, 1024>), g, b, lds, st, I, W, buf, khat, tw); break;
+    case 2048: MMC_LAUNCH((k_ymul_n<T, 2048>), g, b, lds, st, I, W, buf, khat, tw); break;
+    default: MMC_LAUNCH(k_ymul<T>, g, b, lds, st, I, W, buf, khat, tw); break;
+    }
     return post();
 }
 
@@ -914,11 +1106,17 @@ hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W, int 
 }
 
 template <typename T>
-hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
+hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n, int rows,
                        const cpx<T> *spec, T *real, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    MMC_LAUNCH(k_xinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, spec, real, tw);
+    const dim3 g(n), b(kThreads);
+    constexpr int RM = kXRows<T>::value;
+#define MMC_XI_CASE(TT, NN, RR) \
+    if (fixed_n == NN && rows == RR) { MMC_LAUNCH((k_xinv_n<TT, NN, RR>), g, b, lds, st, I, W, spec, real, tw); return post(); }
+    MMC_XFIXED(MMC_XI_CASE, T, 1) MMC_XFIXED(MMC_XI_CASE, T, RM)
+#undef MMC_XI_CASE
+    MMC_LAUNCH(k_xinv<T>, g, b, lds, st, I, W, spec, real, tw);
     return post();
 }
 
@@ -984,9 +1182,23 @@ static hipError_t set_lds_t(int bytes)
     };
     one((const void *)k_xfwd<T>);
     one((const void *)k_xinv<T>);
+    constexpr int RM = kXRows<T>::value;
+#define MMC_XL(TT, NN, RR) one((const void *)k_xfwd_n<TT, NN, RR>); one((const void *)k_xinv_n<TT, NN, RR>);
+    MMC_XFIXED(MMC_XL, T, 1) MMC_XFIXED(MMC_XL, T, RM)
+#undef MMC_XL
     one((const void *)k_zfwd<T>);
     one((const void *)k_zinv<T>);
     one((const void *)k_ymul<T>);
+    one((const void *)k_ymul_n<T, 64>);
+    one((const void *)k_ymul_n<T, 96>);
+    one((const void *)k_ymul_n<T, 128>);
+    one((const void *)k_ymul_n<T, 192>);
+    one((const void *)k_ymul_n<T, 256>);
+    one((const void *)k_ymul_n<T, 384>);
+    one((const void *)k_ymul_n<T, 512>);
+    one((const void *)k_ymul_n<T, 768>);
+    one((const void *)k_ymul_n<T, 1024>);
+    one((const void *)k_ymul_n<T, 2048>);
     one((const void *)k_yfwd_tensor<T>);
     return e;
 }
@@ -999,18 +1211,18 @@ hipError_t set_lds_limit(size_t bytes)
 }
 
 #define MMC_INST(T)                                                                           \
-    template hipError_t launch_xfwd<T>(hipStream_t, const Inst *, const Work *, int, int,     \
-                                       const T *, cpx<T> *, const cpx<T> *, size_t);          \
+    template hipError_t launch_xfwd<T>(hipStream_t, const Inst *, const Work *, int, int, int, \
+                                       int, const T *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_zfwd<T>(hipStream_t, const Inst *, const Work *, int, int,     \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int,          \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
-    template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, cpx<T> *, \
-                                       const cpx<T> *, const cpx<T> *, size_t);               \
+    template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int,     \
+                                       cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_yfwd_tensor<T>(hipStream_t, const Inst *, const Work *, int,   \
                                               const cpx<double> *, cpx<T> *,                  \
                                               const cpx<double> *, size_t);                   \
-    template hipError_t launch_xinv<T>(hipStream_t, const Inst *, const Work *, int,          \
+    template hipError_t launch_xinv<T>(hipStream_t, const Inst *, const Work *, int, int, int, \
                                        const cpx<T> *, T *, const cpx<T> *, size_t);          \
     template hipError_t launch_metro<T>(hipStream_t, const Inst *, const Work *, int, T *,    \
                                         const T *, const double *, const uint32_t *,          \
