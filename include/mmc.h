@@ -186,7 +186,9 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * "graph_sweeps" (sweeps per captured graph, default 16),
  * "timing" (1 = time every launch with hipEvents recorded by its dispatch
  * packet on the context's stream; sweeps then launch eagerly),
- * "tyx" (x-pass rows per workgroup), "ncol" (y-pass columns per workgroup).
+ * "tyx" (x-pass rows per workgroup), "ncol" (y-pass columns per workgroup),
+ * "metro_tiled" (1: both checkerboard colours in one tiled launch where
+ * every instance has a checkerboard; 0, the default: one launch per colour).
  * Tile options only regroup independent transforms: results are unchanged.
  * Unknown keys return MMC_EINVAL. */
 int mmc_set_option(mmc_ctx *ctx, const char *key, long value);

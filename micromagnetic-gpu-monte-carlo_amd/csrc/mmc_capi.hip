@@ -111,7 +111,10 @@ struct mmc_ctx {
     size_t m_elems = 0, a_elems = 0, b_elems = 0, khat_elems = 0, k_elems = 0, ka_elems = 0;
     long maxN = 0;
 
-    List xl, zl, yl, cl, ml, tl, txl, tzl, tyl;   // FFT passes, cells, colour slots, tensor passes
+    List xl, zl, yl, cl, ml, mtl, tl, txl, tzl, tyl;  // FFT passes, cells, colour slots,
+                                                      // Metropolis tiles, tensor passes
+    bool metro_tiled = false;         // both colours per launch (k_metro_tile)
+    bool metro_tiled_ok = false;      // every instance admits the tiled kernel
     int max_colors = 2;
     bool any_demag = false;
     bool tensor_ready = false;
@@ -236,6 +239,27 @@ bool choose_tiles(const mmc_ctx *c, Inst &in, size_t pref, size_t maxb)
     return pairs && in.tyz && in.ncol && in.tyx64 && in.tyz64 && yt;
 }
 
+// LDS of a Metropolis tile: m with a 2-cell halo, h with a 1-cell halo.
+size_t mtile_lds(const mmc_ctx *c, int tx, int ty, int nz)
+{
+    return 3 * (size_t)nz * ((size_t)(tx + 4) * (ty + 4) + (size_t)(tx + 2) * (ty + 2)) * c->tsize;
+}
+
+// Widest Metropolis tile (checkerboard instances only) whose LDS fits the
+// preferred budget; (0,0) when none does.
+void choose_mtile(const mmc_ctx *c, Inst &in)
+{
+    static const int cand[][2] = {{64, 8}, {64, 4}, {32, 8}, {32, 4}, {32, 2}};
+    in.mtx = in.mty = 0;
+    if (in.ncolors != 2) return;
+    for (auto &t : cand)
+        if (mtile_lds(c, t[0], t[1], in.nz) <= c->lds_pref) {
+            in.mtx = t[0];
+            in.mty = t[1];
+            return;
+        }
+}
+
 int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
 {
     if (p.nx < 1 || p.ny < 1 || p.nz < 1)
@@ -272,6 +296,7 @@ int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
         if (in.pbc[a] && n >= 3 && (n & 1)) in.ncolors = 3;
     }
     in.cal_demag = p.cal_demag ? 1 : 0;
+    choose_mtile(c, in);
     // physics in reduced units (fields per Ms, energies per mu0 Ms^2 V)
     const double mu0 = 4e-7 * kPiTrunc;          // src/main.cu:244
     const double ms = p.ms;
@@ -349,7 +374,7 @@ void build_lists(mmc_ctx *c)
     std::map<int, std::vector<Work>> yitems;
     std::map<std::pair<int, int>, Segment> xclass;   // (fixed length, rows) -> items
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
         const bool zdirect = in.Pz <= kZDirect;
@@ -407,7 +432,12 @@ void build_lists(mmc_ctx *c)
         // Metropolis: with two colours a pass visits ceil(nx/2) slots per row
         const long slots = in.ncolors == 2 ? (long)((in.nx + 1) / 2) * in.ny * in.nz : in.N;
         for (long s = 0; s < slots; s += kThreads) c->ml.host.push_back({i, (int)s, 0, 0});
-        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl})
+        if (in.mtx) {
+            for (int y0 = 0; y0 < in.ny; y0 += in.mty)
+                for (int x0 = 0; x0 < in.nx; x0 += in.mtx) c->mtl.host.push_back({i, x0, y0, 0});
+            c->mtl.lds = std::max(c->mtl.lds, mtile_lds(c, in.mtx, in.mty, in.nz));
+        }
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl})
             l->first.push_back(l->n());
     }
     // x and y passes: one contiguous segment (and launch) per kernel variant
@@ -483,6 +513,23 @@ void adapt_tiles(mmc_ctx *c)
         for (auto &in : c->inst)
             if (in.ncol > 1) { in.ncol = std::max(1, in.ncol / 2); changed = true; }
     }
+    auto mitems = [&] {
+        long n = 0;
+        for (auto &in : c->inst)
+            if (in.mtx) n += (long)((in.nx + in.mtx - 1) / in.mtx) * ((in.ny + in.mty - 1) / in.mty);
+        return n;
+    };
+    for (bool changed = true; changed && mitems() < target;) {
+        changed = false;
+        for (auto &in : c->inst) {
+            if (!in.mtx) continue;
+            if (in.mty > 2) { in.mty /= 2; changed = true; }
+            else if (in.mtx > 16) { in.mtx /= 2; changed = true; }
+        }
+    }
+    c->metro_tiled_ok = true;
+    for (auto &in : c->inst) if (!in.mtx) c->metro_tiled_ok = false;
+    c->metro_tiled = false;           // per-colour launches unless asked
 }
 
 int alloc_all(mmc_ctx *c)
@@ -549,7 +596,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
     HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -616,7 +663,7 @@ int retile(mmc_ctx *c, int tyx, int ncol)
         if (ncol > 0) in.ncol = std::max(1, std::min(ncol, max_ncol(c, in)));
     }
     build_lists(c);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -667,6 +714,17 @@ int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
     int rc = demag_chain<T>(c);
     if (rc) return rc;
     hipStream_t st = c->stream;
+    if (c->metro_tiled) {
+        // one launch: m read, h read and m written once per sweep
+        double bytes = 0;
+        for (auto &in : c->inst) bytes += (in.cal_demag ? 9.0 : 6.0) * in.N * c->tsize;
+        HIPCHK(c, timed(c, K_METRO, [&] {
+                   return launch_metro_tile<T>(st, c->dInst, c->mtl.dev, c->mtl.n(), (T *)c->d_m,
+                                               (const T *)c->d_h, c->d_beta, c->d_ctl, sweep_off,
+                                               c->any_demag ? 1 : 0, c->mtl.lds);
+               }, bytes));
+        return MMC_OK;
+    }
     for (int col = 0; col < c->max_colors; ++col)
         HIPCHK(c, timed(c, K_METRO, [&] {
                    return launch_metro<T>(st, c->dInst, c->ml.dev, c->ml.n(), (T *)c->d_m,
@@ -887,7 +945,7 @@ void mmc_destroy(mmc_ctx *c)
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
         if (p) hipFree(p);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->tl, &c->txl, &c->tzl, &c->tyl})
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl})
         if (l->dev) hipFree(l->dev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1211,6 +1269,15 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
         c->graph_sweeps = (int)value;
+        return MMC_OK;
+    }
+    if (k == "metro_tiled") {
+        if (value && !c->metro_tiled_ok)
+            return fail(c, MMC_EUNSUP, "tiled Metropolis needs a checkerboard (even periodic axes) "
+                                       "and a tile that fits LDS for every instance");
+        c->metro_tiled = value != 0;
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
         return MMC_OK;
     }
     if (k == "tyx" || k == "ncol") {

@@ -46,6 +46,7 @@ struct Inst {
     int tyx, tyz, ncol;   // tile shapes of the bulk-precision FFT passes; each
                           // pass ping-pongs two LDS buffers of tile size
     int tyx64, tyz64;     // tile shapes of the FP64 tensor transform
+    int mtx, mty;         // Metropolis x-y tile (both colours in one pass)
     long N, NP;
     long m_off, h_off;    // T
     long a_off, b_off;    // cpx<T>
@@ -115,6 +116,12 @@ hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n,
                         T *m, const T *h, const double *beta,
                         const uint32_t *ctl, uint32_t sweep_off, int color,
                         int use_demag);
+// Both checkerboard colours in one launch; items (inst, x0, y0).
+template <typename T>
+hipError_t launch_metro_tile(hipStream_t st, const Inst *I, const Work *W, int n,
+                             T *m, const T *h, const double *beta,
+                             const uint32_t *ctl, uint32_t sweep_off,
+                             int use_demag, size_t lds);
 template <typename T>
 hipError_t launch_energy(hipStream_t st, const Inst *I, const Work *W, int n,
                          const T *m, const T *h, double *partials,

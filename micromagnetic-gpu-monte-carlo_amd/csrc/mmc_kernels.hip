@@ -692,9 +692,36 @@ __device__ __forceinline__ V3<T> load_m(const T *m, long N, long i)
     return {m[i], m[N + i], m[2 * N + i]};
 }
 
-// Neighbour sums of one cell (reduced units, field per Ms):
+// Bond sums of one cell from its six neighbour values (a missing bond
+// passes zeros), reduced units (field per Ms):
 //   ex  = sum_a lam_a * sum_{j in nbrs_a} m_j      (exchange, neighbours only)
-//   dmi = interfacial DMI bond field
+//   dmi = interfacial DMI bond field (x and y bonds)
+template <typename T>
+__device__ __forceinline__ void bond_sums(const Inst &in, const V3<T> &xp, const V3<T> &xm,
+                                          const V3<T> &yp, const V3<T> &ym, const V3<T> &zp,
+                                          const V3<T> &zm, V3<T> &ex, V3<T> &dmi)
+{
+    ex = {T(0), T(0), T(0)};
+    dmi = {T(0), T(0), T(0)};
+    {   // x bonds
+        const T lam = T(in.lam[0]), dm = T(in.dmi[0]);
+        ex.x += lam * (xp.x + xm.x); ex.y += lam * (xp.y + xm.y); ex.z += lam * (xp.z + xm.z);
+        dmi.x += dm * (xp.z - xm.z);
+        dmi.z -= dm * (xp.x - xm.x);
+    }
+    {   // y bonds
+        const T lam = T(in.lam[1]), dm = T(in.dmi[1]);
+        ex.x += lam * (yp.x + ym.x); ex.y += lam * (yp.y + ym.y); ex.z += lam * (yp.z + ym.z);
+        dmi.y += dm * (yp.z - ym.z);
+        dmi.z -= dm * (yp.y - ym.y);
+    }
+    if (in.nz > 1) {   // z bonds: exchange only
+        const T lam = T(in.lam[2]);
+        ex.x += lam * (zp.x + zm.x); ex.y += lam * (zp.y + zm.y); ex.z += lam * (zp.z + zm.z);
+    }
+}
+
+// Neighbour sums of cell (x,y,z) read from global m, plus
 //   cnt = sum_a lam_a * (#neighbours along a)       (Laplacian diagonal)
 template <typename T>
 __device__ __forceinline__ void stencil(const Inst &in, const T *m, int x, int y, int z,
@@ -702,38 +729,20 @@ __device__ __forceinline__ void stencil(const Inst &in, const T *m, int x, int y
 {
     const long N = in.N;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
-    ex = {T(0), T(0), T(0)};
-    dmi = {T(0), T(0), T(0)};
+    const V3<T> zero = {T(0), T(0), T(0)};
+    V3<T> xp = zero, xm = zero, yp = zero, ym = zero, zp = zero, zm = zero;
     cnt = T(0);
     const long row = (long)nx * (y + (long)ny * z);
-    {   // x bonds
-        const T lam = T(in.lam[0]), dm = T(in.dmi[0]);
-        const int jp = nbr(x, +1, nx, in.pbc[0]), jm = nbr(x, -1, nx, in.pbc[0]);
-        V3<T> p = {T(0), T(0), T(0)}, q = {T(0), T(0), T(0)};
-        if (jp >= 0) { p = load_m(m, N, row + jp); cnt += lam; }
-        if (jm >= 0) { q = load_m(m, N, row + jm); cnt += lam; }
-        ex.x += lam * (p.x + q.x); ex.y += lam * (p.y + q.y); ex.z += lam * (p.z + q.z);
-        dmi.x += dm * (p.z - q.z);
-        dmi.z -= dm * (p.x - q.x);
+    int j;
+    if ((j = nbr(x, +1, nx, in.pbc[0])) >= 0) { xp = load_m(m, N, row + j); cnt += T(in.lam[0]); }
+    if ((j = nbr(x, -1, nx, in.pbc[0])) >= 0) { xm = load_m(m, N, row + j); cnt += T(in.lam[0]); }
+    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) { yp = load_m(m, N, x + (long)nx * (j + (long)ny * z)); cnt += T(in.lam[1]); }
+    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) { ym = load_m(m, N, x + (long)nx * (j + (long)ny * z)); cnt += T(in.lam[1]); }
+    if (nz > 1) {
+        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) { zp = load_m(m, N, x + (long)nx * (y + (long)ny * j)); cnt += T(in.lam[2]); }
+        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) { zm = load_m(m, N, x + (long)nx * (y + (long)ny * j)); cnt += T(in.lam[2]); }
     }
-    {   // y bonds
-        const T lam = T(in.lam[1]), dm = T(in.dmi[1]);
-        const int jp = nbr(y, +1, ny, in.pbc[1]), jm = nbr(y, -1, ny, in.pbc[1]);
-        V3<T> p = {T(0), T(0), T(0)}, q = {T(0), T(0), T(0)};
-        if (jp >= 0) { p = load_m(m, N, x + (long)nx * (jp + (long)ny * z)); cnt += lam; }
-        if (jm >= 0) { q = load_m(m, N, x + (long)nx * (jm + (long)ny * z)); cnt += lam; }
-        ex.x += lam * (p.x + q.x); ex.y += lam * (p.y + q.y); ex.z += lam * (p.z + q.z);
-        dmi.y += dm * (p.z - q.z);
-        dmi.z -= dm * (p.y - q.y);
-    }
-    if (nz > 1) {   // z bonds: exchange only
-        const T lam = T(in.lam[2]);
-        const int jp = nbr(z, +1, nz, in.pbc[2]), jm = nbr(z, -1, nz, in.pbc[2]);
-        V3<T> p = {T(0), T(0), T(0)}, q = {T(0), T(0), T(0)};
-        if (jp >= 0) { p = load_m(m, N, x + (long)nx * (y + (long)ny * jp)); cnt += lam; }
-        if (jm >= 0) { q = load_m(m, N, x + (long)nx * (y + (long)ny * jm)); cnt += lam; }
-        ex.x += lam * (p.x + q.x); ex.y += lam * (p.y + q.y); ex.z += lam * (p.z + q.z);
-    }
+    bond_sums(in, xp, xm, yp, ym, zp, zm, ex, dmi);
 }
 
 template <typename T>
@@ -745,9 +754,43 @@ __device__ __forceinline__ T quad_k0(const Inst &in, V3<T> a)
            T(2) * (k1 * a.x * a.y + k2 * a.x * a.z + k4 * a.y * a.z);
 }
 
-// One colour class.  dE (units mu0 Ms^2 V) =
+// Metropolis decision for one cell.  dE (units mu0 Ms^2 V) =
 //   -d.(ex + dmi + b + h - K0 m) - kap[(m'.u)^2 - (m.u)^2] - [m'K0m' - mK0m]/2
 // accept if dE <= 0 or u < exp(-dE * beta), beta = mu0 Ms^2 V / (kB T).
+// Trial m' uniform on the sphere from Philox counter (cell, sweep, stage, 0).
+template <typename T>
+__device__ __forceinline__ bool metro_cell(const Inst &in, const V3<T> &mi, const V3<T> &ex,
+                                           const V3<T> &dmi, const V3<T> &hi, int use_demag,
+                                           uint32_t cell, uint32_t sweep, uint32_t stage, T bt,
+                                           V3<T> &mn)
+{
+    V3<T> g = {ex.x + dmi.x + T(in.b[0]), ex.y + dmi.y + T(in.b[1]), ex.z + dmi.z + T(in.b[2])};
+    if (use_demag) {
+        const T k0 = T(in.K0[0]), k1 = T(in.K0[1]), k2 = T(in.K0[2]);
+        const T k3 = T(in.K0[3]), k4 = T(in.K0[4]), k5 = T(in.K0[5]);
+        g.x += hi.x - (k0 * mi.x + k1 * mi.y + k2 * mi.z);
+        g.y += hi.y - (k1 * mi.x + k3 * mi.y + k4 * mi.z);
+        g.z += hi.z - (k2 * mi.x + k4 * mi.y + k5 * mi.z);
+    }
+    const uint4 rr = philox10(make_uint4(cell, sweep, stage, 0u), in.key[0], in.key[1]);
+    const T zt = T(1) - T(2) * u01<T>(rr.x);
+    const T st = sqrt_t(max(T(0), T(1) - zt * zt));
+    T sn, cs;
+    sincospi_t(T(2) * u01<T>(rr.y), &sn, &cs);
+    mn = {st * cs, st * sn, zt};
+    const V3<T> d = {mn.x - mi.x, mn.y - mi.y, mn.z - mi.z};
+    T dE = -(d.x * g.x + d.y * g.y + d.z * g.z);
+    const T ux = T(in.u[0]), uy = T(in.u[1]), uz = T(in.u[2]);
+    const T pu = mn.x * ux + mn.y * uy + mn.z * uz;
+    const T ou = mi.x * ux + mi.y * uy + mi.z * uz;
+    dE -= T(in.kap) * (pu * pu - ou * ou);
+    if (use_demag) dE -= T(0.5) * (quad_k0(in, mn) - quad_k0(in, mi));
+    return dE <= T(0) || u01<T>(rr.z) < exp_t(-dE * bt);
+}
+
+// One colour class, one thread per cell of the class (mcKernel2,
+// src/mc_sampling.h:27-41); used when a periodic axis has odd length
+// (three classes) and by the tiled kernel's fallback.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_metro(const Inst *__restrict__ I,
                                                     const Work *__restrict__ W,
@@ -791,34 +834,150 @@ __global__ __launch_bounds__(kThreads) void k_metro(const Inst *__restrict__ I,
     V3<T> ex, dmi;
     T cnt;
     stencil(in, m, x, y, z, ex, dmi, cnt);
-    V3<T> g = {ex.x + dmi.x + T(in.b[0]), ex.y + dmi.y + T(in.b[1]), ex.z + dmi.z + T(in.b[2])};
-    if (use_demag) {
-        const T *h = hb + in.h_off;
-        const T k0 = T(in.K0[0]), k1 = T(in.K0[1]), k2 = T(in.K0[2]);
-        const T k3 = T(in.K0[3]), k4 = T(in.K0[4]), k5 = T(in.K0[5]);
-        g.x += h[i] - (k0 * mi.x + k1 * mi.y + k2 * mi.z);
-        g.y += h[N + i] - (k1 * mi.x + k3 * mi.y + k4 * mi.z);
-        g.z += h[2 * N + i] - (k2 * mi.x + k4 * mi.y + k5 * mi.z);
-    }
-    const uint32_t sweep = ctl[1] + sweep_off;
-    const uint4 rr = philox10(make_uint4((uint32_t)i, sweep, ctl[0], 0u), in.key[0], in.key[1]);
-    const T zt = T(1) - T(2) * u01<T>(rr.x);
-    const T st = sqrt_t(max(T(0), T(1) - zt * zt));
-    T sn, cs;
-    sincospi_t(T(2) * u01<T>(rr.y), &sn, &cs);
-    const V3<T> mn = {st * cs, st * sn, zt};
-    const V3<T> d = {mn.x - mi.x, mn.y - mi.y, mn.z - mi.z};
-    T dE = -(d.x * g.x + d.y * g.y + d.z * g.z);
-    const T ux = T(in.u[0]), uy = T(in.u[1]), uz = T(in.u[2]);
-    const T pu = mn.x * ux + mn.y * uy + mn.z * uz;
-    const T ou = mi.x * ux + mi.y * uy + mi.z * uz;
-    dE -= T(in.kap) * (pu * pu - ou * ou);
-    if (use_demag) dE -= T(0.5) * (quad_k0(in, mn) - quad_k0(in, mi));
-    const T bt = T(beta[w.inst]);
-    if (dE <= T(0) || u01<T>(rr.z) < exp_t(-dE * bt)) {
+    V3<T> hi = {T(0), T(0), T(0)};
+    if (use_demag) hi = load_m(hb + in.h_off, N, i);
+    V3<T> mn;
+    if (metro_cell(in, mi, ex, dmi, hi, use_demag, (uint32_t)i, ctl[1] + sweep_off, ctl[0],
+                   T(beta[w.inst]), mn)) {
         m[i] = mn.x;
         m[N + i] = mn.y;
         m[2 * N + i] = mn.z;
+    }
+}
+
+// Both colour classes of a checkerboard in one pass.  Item (inst, x0, y0):
+// an x-y tile of mtx x mty cells through all z layers.  LDS holds m with a
+// two-cell x-y halo and h with a one-cell halo (periodic axes wrap, open
+// edges are absent).  Colour 0 is updated on the tile plus a one-cell ring
+// (the ring cells are the neighbours colour 1 needs; their updates repeat
+// exactly what the neighbouring tile computes, same inputs and Philox
+// counter), then colour 1 on the tile, and the tile is written back once.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_metro_tile(const Inst *__restrict__ I,
+                                                         const Work *__restrict__ W,
+                                                         T *__restrict__ mb,
+                                                         const T *__restrict__ hb,
+                                                         const double *__restrict__ beta,
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint32_t sweep_off, int use_demag)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const int x0 = w.a, y0 = w.b;
+    const int TX = in.mtx, TY = in.mty;
+    const int MW = TX + 4, MH = TY + 4;          // m region: halo 2
+    const int HW = TX + 2, HH = TY + 2;          // h region: halo 1
+    const int mplane = MW * MH * nz, hplane = HW * HH * nz;
+    T *sm = reinterpret_cast<T *>(g_smem);       // [3][nz][MH][MW]
+    T *sh = sm + 3 * mplane;                     // [3][nz][HH][HW]
+    const long N = in.N;
+    const T *m = mb + in.m_off;
+    const T *h = hb + in.h_off;
+    const int px_ = in.pbc[0], py_ = in.pbc[1];
+    // wrap a raw coordinate into [0,n) (periodic) or flag it absent (-1)
+    auto wrap = [](int g, int n, int periodic) -> int {
+        if (g >= 0 && g < n) return g;
+        if (!periodic) return -1;
+        if (g < 0 && g + n >= 0) return g + n;
+        if (g >= n && g - n < n) return g - n;
+        g %= n;
+        return g < 0 ? g + n : g;
+    };
+    for (int e = threadIdx.x; e < mplane; e += kThreads) {
+        const int xx = e % MW, t = e / MW, yy = t % MH, z = t / MH;
+        const int gx = wrap(x0 - 2 + xx, nx, px_);
+        const int gy = wrap(y0 - 2 + yy, ny, py_);
+        V3<T> v = {T(0), T(0), T(0)};
+        if (gx >= 0 && gy >= 0) v = load_m(m, N, gx + (long)nx * (gy + (long)ny * z));
+        sm[e] = v.x;
+        sm[mplane + e] = v.y;
+        sm[2 * mplane + e] = v.z;
+    }
+    if (use_demag)
+        for (int e = threadIdx.x; e < hplane; e += kThreads) {
+            const int xx = e % HW, t = e / HW, yy = t % HH, z = t / HH;
+            const int gx = wrap(x0 - 1 + xx, nx, px_);
+            const int gy = wrap(y0 - 1 + yy, ny, py_);
+            V3<T> v = {T(0), T(0), T(0)};
+            if (gx >= 0 && gy >= 0) v = load_m(h, N, gx + (long)nx * (gy + (long)ny * z));
+            sh[e] = v.x;
+            sh[hplane + e] = v.y;
+            sh[2 * hplane + e] = v.z;
+        }
+    __syncthreads();
+    const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
+    const T bt = T(beta[w.inst]);
+    // a bond along an axis exists unless the cell sits on an open edge; a
+    // periodic axis of length 1 has none (the cell would be its own neighbour)
+    const bool xper = px_ && nx > 1, yper = py_ && ny > 1;
+    // update the cell at m-region offset o (position px,py,z); global (gx,gy,z)
+    auto update = [&](int o, int gx, int gy, int z) {
+        const V3<T> zero = {T(0), T(0), T(0)};
+        V3<T> xp = zero, xm = zero, yp = zero, ym = zero, zp = zero, zm = zero;
+        auto at = [&](int q) -> V3<T> { return {sm[q], sm[mplane + q], sm[2 * mplane + q]}; };
+        if (xper || gx + 1 < nx) xp = at(o + 1);
+        if (xper || gx > 0) xm = at(o - 1);
+        if (yper || gy + 1 < ny) yp = at(o + MW);
+        if (yper || gy > 0) ym = at(o - MW);
+        if (nz > 1) {
+            int j;
+            if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) zp = at(o + (j - z) * MW * MH);
+            if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) zm = at(o + (j - z) * MW * MH);
+        }
+        V3<T> ex, dmi;
+        bond_sums(in, xp, xm, yp, ym, zp, zm, ex, dmi);
+        const V3<T> mi = at(o);
+        V3<T> hi = zero;
+        if (use_demag) {
+            // h region is the m region shrunk by one cell on every side
+            const int ph = o % MW, rest = o / MW, qy = rest % MH, qz = rest / MH;
+            const int oh = (ph - 1) + HW * ((qy - 1) + HH * qz);
+            hi = {sh[oh], sh[hplane + oh], sh[2 * hplane + oh]};
+        }
+        V3<T> mn;
+        const uint32_t cell = (uint32_t)(gx + (long)nx * (gy + (long)ny * z));
+        if (metro_cell(in, mi, ex, dmi, hi, use_demag, cell, sweep, stage, bt, mn)) {
+            sm[o] = mn.x;
+            sm[mplane + o] = mn.y;
+            sm[2 * mplane + o] = mn.z;
+        }
+    };
+    // colour 0 on the tile and its one-cell ring: in ring row (yy, z) the
+    // colour-0 positions are xx = 2k + ((x0 + y0 + yy + z) & 1), xx < HW
+    {
+        const int hc = (HW + 1) >> 1;
+        for (int e = threadIdx.x; e < hc * HH * nz; e += kThreads) {
+            const int k = e % hc, r = e / hc, yy = r % HH, z = r / HH;
+            const int xx = 2 * k + ((x0 + y0 + yy + z) & 1);
+            if (xx >= HW) continue;
+            const int gx = wrap(x0 - 1 + xx, nx, px_);
+            const int gy = wrap(y0 - 1 + yy, ny, py_);
+            if (gx < 0 || gy < 0) continue;
+            update((xx + 1) + MW * ((yy + 1) + MH * z), gx, gy, z);
+        }
+    }
+    __syncthreads();
+    // colour 1 on the tile: xx = 2k + ((x0 + y0 + yy + z + 1) & 1), xx < tw
+    const int tw = min(TX, nx - x0), th = min(TY, ny - y0);
+    {
+        const int hc = (tw + 1) >> 1;
+        for (int e = threadIdx.x; e < hc * th * nz; e += kThreads) {
+            const int k = e % hc, r = e / hc, yy = r % th, z = r / th;
+            const int xx = 2 * k + ((x0 + y0 + yy + z + 1) & 1);
+            if (xx >= tw) continue;
+            update((xx + 2) + MW * ((yy + 2) + MH * z), x0 + xx, y0 + yy, z);
+        }
+    }
+    __syncthreads();
+    T *mo = mb + in.m_off;
+    for (int e = threadIdx.x; e < tw * th * nz; e += kThreads) {
+        const int xx = e % tw, t = e / tw, yy = t % th, z = t / th;
+        const long gi = (x0 + xx) + (long)nx * ((y0 + yy) + (long)ny * z);
+        const int o = (xx + 2) + MW * ((yy + 2) + MH * z);
+        mo[gi] = sm[o];
+        mo[N + gi] = sm[mplane + o];
+        mo[2 * N + gi] = sm[2 * mplane + o];
     }
 }
 
@@ -1132,6 +1291,17 @@ hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n, T *
 }
 
 template <typename T>
+hipError_t launch_metro_tile(hipStream_t st, const Inst *I, const Work *W, int n, T *m, const T *h,
+                             const double *beta, const uint32_t *ctl, uint32_t sweep_off,
+                             int use_demag, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    MMC_LAUNCH(k_metro_tile<T>, dim3(n), dim3(kThreads), lds, st, I, W, m, h, beta, ctl, sweep_off,
+               use_demag);
+    return post();
+}
+
+template <typename T>
 hipError_t launch_energy(hipStream_t st, const Inst *I, const Work *W, int n, const T *m,
                          const T *h, double *partials, int use_demag)
 {
@@ -1200,6 +1370,7 @@ static hipError_t set_lds_t(int bytes)
     one((const void *)k_ymul_n<T, 1024>);
     one((const void *)k_ymul_n<T, 2048>);
     one((const void *)k_yfwd_tensor<T>);
+    one((const void *)k_metro_tile<T>);
     return e;
 }
 
@@ -1227,6 +1398,9 @@ hipError_t set_lds_limit(size_t bytes)
     template hipError_t launch_metro<T>(hipStream_t, const Inst *, const Work *, int, T *,    \
                                         const T *, const double *, const uint32_t *,          \
                                         uint32_t, int, int);                                  \
+    template hipError_t launch_metro_tile<T>(hipStream_t, const Inst *, const Work *, int,    \
+                                             T *, const T *, const double *, const uint32_t *, \
+                                             uint32_t, int, size_t);                          \
     template hipError_t launch_energy<T>(hipStream_t, const Inst *, const Work *, int,        \
                                          const T *, const T *, double *, int);                \
     template hipError_t launch_field<T>(hipStream_t, const Inst *, const Work *, int,         \
