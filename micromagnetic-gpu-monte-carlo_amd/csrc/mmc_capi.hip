@@ -32,7 +32,7 @@ enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY 
 
 struct Segment {
     int fixed_n = 0;   // common FFT length of the segment's kernel, 0 = generic
-    int rows = 0;      // x pass: row pairs per fixed-plan tile
+    int rows = 0;      // row pairs (x pass) or columns (y pass) per fixed-plan tile
     int begin = 0, count = 0;
     size_t lds = 0;
     double bytes = 0;  // algorithmic HBM bytes of one launch of the segment
@@ -370,8 +370,8 @@ void build_lists(mmc_ctx *c)
 {
     const size_t cs = 2 * c->tsize;   // complex element size
     auto reset = [&](List &l) { l.host.clear(); l.first.assign(1, 0); l.lds = 0; l.seg.clear(); };
-    std::map<int, Segment> yclass;           // fixed length (0: generic) -> items
-    std::map<int, std::vector<Work>> yitems;
+    std::map<std::pair<int, int>, Segment> yclass;   // (fixed length or 0, columns) -> items
+    std::map<std::pair<int, int>, std::vector<Work>> yitems;
     std::map<std::pair<int, int>, Segment> xclass;   // (fixed length, rows) -> items
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
@@ -401,13 +401,15 @@ void build_lists(mmc_ctx *c)
                 if (!zdirect) c->zl.lds = std::max(c->zl.lds, (2 * (size_t)in.tyz + 1) * in.Pz * cs);
             }
             const int fn = fixed_fft_size(in.Py) ? in.Py : 0;
+            const std::pair<int, int> yk(fn, fn ? in.ncol : 0);
             for (int kz = 0; kz < in.Pz; ++kz)
                 for (int kx0 = 0; kx0 < in.Hx; kx0 += in.ncol)
-                    yitems[fn].push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
-            const size_t ylds = fn ? (3 * (size_t)fixed_cols(in.Py) + 1) * in.Py * cs
+                    yitems[yk].push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
+            const size_t ylds = fn ? (3 * (size_t)in.ncol + 1) * in.Py * cs
                                    : (2 * (size_t)3 * in.ncol + 1) * in.Py * cs;
-            Segment &sg = yclass[fn];
+            Segment &sg = yclass[yk];
             sg.fixed_n = fn;
+            sg.rows = yk.second;
             sg.lds = std::max(sg.lds, ylds);
             sg.bytes += inst_bytes(c, in, K_YMUL);
             c->yl.lds = std::max(c->yl.lds, ylds);
@@ -511,7 +513,11 @@ void adapt_tiles(mmc_ctx *c)
     for (bool changed = true; changed && yitems() < target;) {
         changed = false;
         for (auto &in : c->inst)
-            if (in.ncol > 1) { in.ncol = std::max(1, in.ncol / 2); changed = true; }
+            if (in.ncol > 1) {
+                // fixed-plan y items come in two widths: fixed_cols(Py), or one
+                in.ncol = fixed_fft_size(in.Py) ? 1 : std::max(1, in.ncol / 2);
+                changed = true;
+            }
     }
     auto mitems = [&] {
         long n = 0;
@@ -660,7 +666,10 @@ int retile(mmc_ctx *c, int tyx, int ncol)
             in.tyx = std::max(2, std::min(tyx & ~1, max_tyx(c, in)));
             if (xfixed(c, in)) in.tyx = in.tyx >= 2 * xrows_for(c->tsize) ? 2 * xrows_for(c->tsize) : 2;
         }
-        if (ncol > 0) in.ncol = std::max(1, std::min(ncol, max_ncol(c, in)));
+        if (ncol > 0) {
+            in.ncol = std::max(1, std::min(ncol, max_ncol(c, in)));
+            if (fixed_fft_size(in.Py) && in.ncol < fixed_cols(in.Py)) in.ncol = 1;
+        }
     }
     build_lists(c);
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
@@ -695,7 +704,7 @@ int demag_chain(mmc_ctx *c)
         HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(st, c->dInst, c->zl.dev, c->zl.n(), 0, A, B, tw, c->zl.lds); }));
     for (const Segment &sg : c->yl.seg)
         HIPCHK(c, timed(c, K_YMUL, [&] {
-                   return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, B, Kh,
+                   return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, B, Kh,
                                          tw, sg.lds);
                }, sg.bytes));
     if (c->zl.n())

@@ -101,7 +101,7 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n,
 // or 0 for the generic kernel.
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
-                       int fixed_n, cpx<T> *buf, const cpx<T> *khat,
+                       int fixed_n, int cols, cpx<T> *buf, const cpx<T> *khat,
                        const cpx<T> *tw, size_t lds);
 template <typename TO>
 hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W,

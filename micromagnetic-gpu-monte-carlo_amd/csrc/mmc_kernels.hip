@@ -526,15 +526,15 @@ __device__ __forceinline__ void kmul_conj(cpx<T> *px, cpx<T> *py, cpx<T> *pz, co
     *pz = cconj(hz);
 }
 
-// Compile-time-planned y pass for Py = N: one LDS buffer of 3*C rows
-// (row comp*C + col), the item's Khat values prefetched into registers
-// before the forward transform so their HBM latency overlaps it.
-template <typename T, int N>
+// Compile-time-planned y pass for Py = N over up to C columns: one LDS
+// buffer of 3*C rows (row comp*C + col), the item's Khat values prefetched
+// into registers before the forward transform so their HBM latency overlaps
+// it.
+template <typename T, int N, int C>
 __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T> *__restrict__ bufb,
                                            const cpx<T> *__restrict__ khatb,
                                            const cpx<T> *__restrict__ twb)
 {
-    constexpr int C = fixed_cols(N);
     constexpr int ROWS = 3 * C;
     constexpr int Q = (C * N + kThreads - 1) / kThreads;
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
@@ -560,9 +560,14 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     for (int e = threadIdx.x; e < ROWS * N; e += kThreads) {
         const int r = e / N, y = e % N;
         const int comp = r / C, col = r % C;
-        cpx<T> v = {T(0), T(0)};
-        if (col < ncol && y < Ly) v = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
-        s[e] = v;
+        // component-wise select: a conditionally loaded struct is kept in scratch
+        T vx = T(0), vy = T(0);
+        if (col < ncol && y < Ly) {
+            const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+            vx = t.x;
+            vy = t.y;
+        }
+        s[e] = {vx, vy};
     }
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
@@ -585,9 +590,11 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     }
 }
 
-// Fixed-plan y pass: every item's instance has Py == N (one launch per
-// distinct length; a kernel of its own keeps its register count low).
-template <typename T, int N>
+// Fixed-plan y pass: every item's instance has Py == N and at most C
+// columns (one launch per distinct (length, C); a kernel of its own keeps its
+// register count low).  C is fixed_cols(N), or 1 for grids too small to fill
+// the chip with wide items.
+template <typename T, int N, int C>
 __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
                                                      const Work *__restrict__ W,
                                                      cpx<T> *__restrict__ bufb,
@@ -595,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
                                                      const cpx<T> *__restrict__ twb)
 {
     const Work w = W[blockIdx.x];
-    ymul_fixed<T, N>(I[w.inst], w, bufb, khatb, twb);
+    ymul_fixed<T, N, C>(I[w.inst], w, bufb, khatb, twb);
 }
 
 // y forward, symmetric 3x3 product with Khat, y inverse.  Item (inst, kx0,
@@ -623,9 +630,13 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
     for (int e = threadIdx.x; e < rows * Py; e += kThreads) {
         const int r = e / Py, y = e - r * Py;
         const int comp = r / ncol, col = r - comp * ncol;
-        cpx<T> v = {T(0), T(0)};
-        if (y < Ly) v = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
-        s[e] = v;
+        T vx = T(0), vy = T(0);
+        if (y < Ly) {
+            const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+            vx = t.x;
+            vy = t.y;
+        }
+        s[e] = {vx, vy};
     }
     __syncthreads();
     s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, tw);
@@ -1234,23 +1245,20 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, cons
 
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
-                       cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+                       int cols, cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
     const dim3 g(n), b(kThreads);
-    switch (fixed_n) {
-    case 64: MMC_LAUNCH((k_ymul_n<T, 64>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 96: MMC_LAUNCH((k_ymul_n<T, 96>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 128: MMC_LAUNCH((k_ymul_n<T, 128>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 192: MMC_LAUNCH((k_ymul_n<T, 192>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 256: MMC_LAUNCH((k_ymul_n<T, 256>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 384: MMC_LAUNCH((k_ymul_n<T, 384>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 512: MMC_LAUNCH((k_ymul_n<T, 512>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 768: MMC_LAUNCH((k_ymul_n<T, 768>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 1024: MMC_LAUNCH((k_ymul_n<T, 1024>), g, b, lds, st, I, W, buf, khat, tw); break;
-    case 2048: MMC_LAUNCH((k_ymul_n<T, 2048>), g, b, lds, st, I, W, buf, khat, tw); break;
-    default: MMC_LAUNCH(k_ymul<T>, g, b, lds, st, I, W, buf, khat, tw); break;
+#define MMC_YN(NN)                                                                              \
+    if (fixed_n == NN) {                                                                        \
+        if (cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1>), g, b, lds, st, I, W, buf, khat, tw);   \
+        else MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN)>), g, b, lds, st, I, W, buf, khat, tw); \
+        return post();                                                                          \
     }
+    MMC_YN(64) MMC_YN(96) MMC_YN(128) MMC_YN(192) MMC_YN(256) MMC_YN(384) MMC_YN(512)
+    MMC_YN(768) MMC_YN(1024) MMC_YN(2048)
+#undef MMC_YN
+    MMC_LAUNCH(k_ymul<T>, g, b, lds, st, I, W, buf, khat, tw);
     return post();
 }
 
@@ -1359,16 +1367,26 @@ static hipError_t set_lds_t(int bytes)
     one((const void *)k_zfwd<T>);
     one((const void *)k_zinv<T>);
     one((const void *)k_ymul<T>);
-    one((const void *)k_ymul_n<T, 64>);
-    one((const void *)k_ymul_n<T, 96>);
-    one((const void *)k_ymul_n<T, 128>);
-    one((const void *)k_ymul_n<T, 192>);
-    one((const void *)k_ymul_n<T, 256>);
-    one((const void *)k_ymul_n<T, 384>);
-    one((const void *)k_ymul_n<T, 512>);
-    one((const void *)k_ymul_n<T, 768>);
-    one((const void *)k_ymul_n<T, 1024>);
-    one((const void *)k_ymul_n<T, 2048>);
+    one((const void *)k_ymul_n<T, 64, 1>);
+    one((const void *)k_ymul_n<T, 64, fixed_cols(64)>);
+    one((const void *)k_ymul_n<T, 96, 1>);
+    one((const void *)k_ymul_n<T, 96, fixed_cols(96)>);
+    one((const void *)k_ymul_n<T, 128, 1>);
+    one((const void *)k_ymul_n<T, 128, fixed_cols(128)>);
+    one((const void *)k_ymul_n<T, 192, 1>);
+    one((const void *)k_ymul_n<T, 192, fixed_cols(192)>);
+    one((const void *)k_ymul_n<T, 256, 1>);
+    one((const void *)k_ymul_n<T, 256, fixed_cols(256)>);
+    one((const void *)k_ymul_n<T, 384, 1>);
+    one((const void *)k_ymul_n<T, 384, fixed_cols(384)>);
+    one((const void *)k_ymul_n<T, 512, 1>);
+    one((const void *)k_ymul_n<T, 512, fixed_cols(512)>);
+    one((const void *)k_ymul_n<T, 768, 1>);
+    one((const void *)k_ymul_n<T, 768, fixed_cols(768)>);
+    one((const void *)k_ymul_n<T, 1024, 1>);
+    one((const void *)k_ymul_n<T, 1024, fixed_cols(1024)>);
+    one((const void *)k_ymul_n<T, 2048, 1>);
+    one((const void *)k_ymul_n<T, 2048, fixed_cols(2048)>);
     one((const void *)k_yfwd_tensor<T>);
     one((const void *)k_metro_tile<T>);
     return e;
@@ -1388,7 +1406,7 @@ hipError_t set_lds_limit(size_t bytes)
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int,          \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
-    template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int,     \
+    template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                        cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_yfwd_tensor<T>(hipStream_t, const Inst *, const Work *, int,   \
                                               const cpx<double> *, cpx<T> *,                  \
