@@ -188,14 +188,22 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * packet on the context's stream; sweeps then launch eagerly),
  * "tyx" (x-pass rows per workgroup), "ncol" (y-pass columns per workgroup),
  * "metro_tiled" (1: both checkerboard colours in one tiled launch where
- * every instance has a checkerboard; 0, the default: one launch per colour).
- * Tile options only regroup independent transforms: results are unchanged.
+ * every instance has a checkerboard; 0, the default: one launch per colour),
+ * "fuse" (1, the default where every instance has demag, a checkerboard and
+ * an x length of 128, 192, 256, 384, 512, 1024 or 2048 after padding: the
+ * inverse x pass runs with colour 0 and colour 1 with the next forward x
+ * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
+ * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
+ * tile, 2048 FP32 / 1024 FP64 complex per component over the x length).
+ * Tile and fusion options only regroup independent work: results are
+ * bit-identical.
  * Unknown keys return MMC_EINVAL. */
 int mmc_set_option(mmc_ctx *ctx, const char *key, long value);
 
 /* Launches, summed device time [ms] and summed algorithmic HBM bytes of
  * one kernel kind ("xfwd", "zfwd", "ymul", "zinv", "xinv", "metro",
- * "energy", "tensor") recorded while "timing" was on; total_bytes /
+ * "energy", "tensor", "xinv_metro", "metro_xfwd") recorded while "timing"
+ * was on; total_bytes /
  * total_ms is the kind's achieved bandwidth.  Resets the counters when
  * reset != 0.  Instrumentation only; no reference counterpart (the
  * reference prints a CPU-clock "stages/s", src/main.cu:795-827). */

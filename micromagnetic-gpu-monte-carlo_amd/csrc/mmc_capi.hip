@@ -26,9 +26,10 @@ namespace {
 constexpr double kPiTrunc = 3.14159265358;       // src/constant.h:21
 constexpr double kBoltzmann = 1.3806488E-23;     // src/constant.h:22
 
-const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy"};
-constexpr int kNumKinds = 8;
-enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY };
+const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy",
+                        "xinv_metro", "metro_xfwd"};
+constexpr int kNumKinds = 10;
+enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY, K_XINVM, K_MXFWD };
 
 struct Segment {
     int fixed_n = 0;   // common FFT length of the segment's kernel, 0 = generic
@@ -113,8 +114,11 @@ struct mmc_ctx {
 
     List xl, zl, yl, cl, ml, mtl, tl, txl, tzl, tyl;  // FFT passes, cells, colour slots,
                                                       // Metropolis tiles, tensor passes
+    List fl;                          // fused x pass + Metropolis tiles
     bool metro_tiled = false;         // both colours per launch (k_metro_tile)
     bool metro_tiled_ok = false;      // every instance admits the tiled kernel
+    bool fuse = false;                // sweeps run the fused x + Metropolis passes
+    bool fuse_ok = false;             // every instance admits them
     int max_colors = 2;
     bool any_demag = false;
     bool tensor_ready = false;
@@ -172,7 +176,7 @@ hipEvent_t next_event(mmc_ctx *c)
 void harvest(mmc_ctx *c)
 {
     if (c->recs.empty()) return;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (auto &r : c->recs) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
@@ -319,7 +323,7 @@ int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
 
 int upload_list(mmc_ctx *c, List &l)
 {
-    if (l.dev) { hipFree(l.dev); l.dev = nullptr; }
+    if (l.dev) { (void)hipFree(l.dev); l.dev = nullptr; }
     if (l.host.empty()) return MMC_OK;
     HIPCHK(c, hipMalloc(&l.dev, sizeof(Work) * l.host.size()));
     HIPCHK(c, hipMemcpy(l.dev, l.host.data(), sizeof(Work) * l.host.size(), hipMemcpyHostToDevice));
@@ -346,6 +350,10 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
         return 3 * N * b + (in.cal_demag ? 3 * N * b / K : 0.0) + 3 * N * b / K;
     }
     case K_ENERGY: return 3 * N * b + (in.cal_demag ? 3 * N * b : 0.0);
+    // x pass (spectrum) + one checkerboard colour: m read (cell and
+    // neighbours), half of m written, half of h written (xinv_metro) or
+    // read (metro_xfwd)
+    case K_XINVM: case K_MXFWD: return spec_x + 6 * N * b;
     case K_TENSOR: return 6.0 * in.NP * 8;
     }
     return 0.0;
@@ -366,6 +374,19 @@ bool xfixed(const mmc_ctx *c, const Inst &in)
            (size_t)(xrows_for(c->tsize) + 1) * in.Px * 2 * c->tsize <= c->lds_max;
 }
 
+// Row pairs of a full fused x + Metropolis tile of instance `in` (0 when the
+// instance cannot take the fused passes: no demag, three colours, an x
+// length without a fixed plan, or LDS too small for one row pair).
+int fused_full(const mmc_ctx *c, const Inst &in)
+{
+    if (!in.cal_demag || in.ncolors != 2 || !xfixed_size(in.Px)) return 0;
+    const size_t cs = 2 * c->tsize;
+    const int r = fused_rows(c->tsize == 4 ? fused_rows_base<float>() : fused_rows_base<double>(),
+                             in.Px);
+    if ((3 * (size_t)r + 1) * in.Px * cs <= c->lds_max) return r;
+    return (4 * (size_t)in.Px * cs <= c->lds_max) ? 1 : 0;
+}
+
 void build_lists(mmc_ctx *c)
 {
     const size_t cs = 2 * c->tsize;   // complex element size
@@ -374,7 +395,9 @@ void build_lists(mmc_ctx *c)
     std::map<std::pair<int, int>, std::vector<Work>> yitems;
     std::map<std::pair<int, int>, Segment> xclass;   // (fixed length, rows) -> items
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) reset(*l);
+    std::map<std::pair<int, int>, Segment> fclass;   // (x length, row pairs) -> items
+    std::map<std::pair<int, int>, std::vector<Work>> fitems;
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl}) reset(*l);
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
         const bool zdirect = in.Pz <= kZDirect;
@@ -439,8 +462,25 @@ void build_lists(mmc_ctx *c)
                 for (int x0 = 0; x0 < in.nx; x0 += in.mtx) c->mtl.host.push_back({i, x0, y0, 0});
             c->mtl.lds = std::max(c->mtl.lds, mtile_lds(c, in.mtx, in.mty, in.nz));
         }
-        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl})
+        if (in.tyf) {
+            const std::pair<int, int> fk(in.Px, in.tyf);
+            for (int z = 0; z < in.nz; ++z)
+                for (int y0 = 0; y0 < in.ny; y0 += 2 * in.tyf) fitems[fk].push_back({i, 0, z, y0});
+            Segment &fs = fclass[fk];
+            fs.fixed_n = in.Px;
+            fs.rows = in.tyf;
+            fs.lds = std::max(fs.lds, (3 * (size_t)in.tyf + 1) * in.Px * cs);
+            fs.bytes += inst_bytes(c, in, K_XINVM);
+        }
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl})
             l->first.push_back(l->n());
+    }
+    for (auto &kv : fitems) {
+        Segment sg = fclass[kv.first];
+        sg.begin = c->fl.n();
+        sg.count = (int)kv.second.size();
+        c->fl.host.insert(c->fl.host.end(), kv.second.begin(), kv.second.end());
+        c->fl.seg.push_back(sg);
     }
     // x and y passes: one contiguous segment (and launch) per kernel variant
     for (auto &kv : xitems) {
@@ -483,59 +523,84 @@ int max_ncol(const mmc_ctx *c, const Inst &in)
     return fit_tw(64, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, c->lds_max, c->lds_max);
 }
 
-// Small grids cannot fill the chip with wide tiles: halve the x-pass row
-// tiles and y-pass column groups until each pass has about two workgroups
-// per compute unit (or the tiles are minimal).
+// Shrink the tiles of every group of instances that one launch serves
+// (same kernel variant) until the group has `target` work items or its
+// tiles are minimal.  key(in) < 0: the instance is not in this pass.
+template <typename Key, typename Items, typename Shrink>
+void adapt_groups(std::vector<Inst> &inst, long target, Key key, Items items, Shrink shrink)
+{
+    std::map<int, std::vector<Inst *>> groups;
+    for (auto &in : inst) {
+        const int k = key(in);
+        if (k >= 0) groups[k].push_back(&in);
+    }
+    for (auto &g : groups)
+        for (bool changed = true; changed;) {
+            long n = 0;
+            for (const Inst *p : g.second) n += items(*p);
+            if (n >= target) break;
+            changed = false;
+            for (Inst *p : g.second) changed |= shrink(*p);
+        }
+}
+
+// Small grids cannot fill the chip with wide tiles: per launch (each
+// fixed-plan length has its own), shrink x-pass row tiles, y-pass column
+// groups, Metropolis tiles and fused tiles until the launch has about two
+// workgroups per compute unit (or the tiles are minimal).
 void adapt_tiles(mmc_ctx *c)
 {
     const long target = 2L * c->num_cu;
-    auto xitems = [&] {
-        long n = 0;
-        for (auto &in : c->inst)
-            if (in.cal_demag) n += 3L * in.nz * ((in.ny + in.tyx - 1) / in.tyx);
-        return n;
-    };
-    auto yitems = [&] {
-        long n = 0;
-        for (auto &in : c->inst)
-            if (in.cal_demag) n += (long)in.Pz * ((in.Hx + in.ncol - 1) / in.ncol);
-        return n;
-    };
-    for (bool changed = true; changed && xitems() < target;) {
-        changed = false;
-        for (auto &in : c->inst)
-            if (in.tyx > 2) {
-                // fixed-plan x tiles come in two shapes: full, or one row pair
-                in.tyx = xfixed(c, in) ? 2 : std::max(2, (in.tyx / 2) & ~1);
-                changed = true;
-            }
-    }
-    for (bool changed = true; changed && yitems() < target;) {
-        changed = false;
-        for (auto &in : c->inst)
-            if (in.ncol > 1) {
-                // fixed-plan y items come in two widths: fixed_cols(Py), or one
-                in.ncol = fixed_fft_size(in.Py) ? 1 : std::max(1, in.ncol / 2);
-                changed = true;
-            }
-    }
-    auto mitems = [&] {
-        long n = 0;
-        for (auto &in : c->inst)
-            if (in.mtx) n += (long)((in.nx + in.mtx - 1) / in.mtx) * ((in.ny + in.mty - 1) / in.mty);
-        return n;
-    };
-    for (bool changed = true; changed && mitems() < target;) {
-        changed = false;
-        for (auto &in : c->inst) {
-            if (!in.mtx) continue;
-            if (in.mty > 2) { in.mty /= 2; changed = true; }
-            else if (in.mtx > 16) { in.mtx /= 2; changed = true; }
-        }
-    }
+    adapt_groups(
+        c->inst, target,
+        [&](const Inst &in) { return in.cal_demag ? (xfixed(c, in) ? in.Px : 0) : -1; },
+        [](const Inst &in) { return 3L * in.nz * ((in.ny + in.tyx - 1) / in.tyx); },
+        [&](Inst &in) {
+            if (in.tyx <= 2) return false;
+            // fixed-plan x tiles come in two shapes: full, or one row pair
+            in.tyx = xfixed(c, in) ? 2 : std::max(2, (in.tyx / 2) & ~1);
+            return true;
+        });
+    adapt_groups(
+        c->inst, target,
+        [](const Inst &in) { return in.cal_demag ? (fixed_fft_size(in.Py) ? in.Py : 0) : -1; },
+        [](const Inst &in) { return (long)in.Pz * ((in.Hx + in.ncol - 1) / in.ncol); },
+        [](Inst &in) {
+            if (in.ncol <= 1) return false;
+            // fixed-plan y items come in two widths: fixed_cols(Py), or one
+            in.ncol = fixed_fft_size(in.Py) ? 1 : std::max(1, in.ncol / 2);
+            return true;
+        });
+    adapt_groups(
+        c->inst, target, [](const Inst &in) { return in.mtx ? 0 : -1; },
+        [](const Inst &in) {
+            return (long)((in.nx + in.mtx - 1) / in.mtx) * ((in.ny + in.mty - 1) / in.mty);
+        },
+        [](Inst &in) {
+            if (in.mty > 2) { in.mty /= 2; return true; }
+            if (in.mtx > 16) { in.mtx /= 2; return true; }
+            return false;
+        });
     c->metro_tiled_ok = true;
     for (auto &in : c->inst) if (!in.mtx) c->metro_tiled_ok = false;
     c->metro_tiled = false;           // per-colour launches unless asked
+    // fused x + Metropolis tiles: full, or one row pair for small grids
+    c->fuse_ok = !c->inst.empty();
+    for (auto &in : c->inst) {
+        in.tyf = fused_full(c, in);
+        if (!in.tyf) c->fuse_ok = false;
+    }
+    if (!c->fuse_ok)
+        for (auto &in : c->inst) in.tyf = 0;
+    adapt_groups(
+        c->inst, target, [](const Inst &in) { return in.tyf ? in.Px : -1; },
+        [](const Inst &in) { return (long)in.nz * ((in.ny + 2 * in.tyf - 1) / (2 * in.tyf)); },
+        [](Inst &in) {
+            if (in.tyf <= 1) return false;
+            in.tyf = 1;
+            return true;
+        });
+    c->fuse = c->fuse_ok;
 }
 
 int alloc_all(mmc_ctx *c)
@@ -602,7 +667,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
     HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -672,7 +737,7 @@ int retile(mmc_ctx *c, int tyx, int ncol)
         }
     }
     build_lists(c);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -684,45 +749,97 @@ int retile(mmc_ctx *c, int tyx, int ncol)
 
 // ---- demag chain -------------------------------------------------------------
 template <typename T>
-int demag_chain(mmc_ctx *c)
+struct Bufs {
+    const cpx<T> *tw;
+    T *m, *h;
+    cpx<T> *A, *B;
+    const cpx<T> *Kh;
+    explicit Bufs(const mmc_ctx *c)
+        : tw((const cpx<T> *)(c->tsize == 4 ? c->d_tw32 : c->d_tw64)), m((T *)c->d_m),
+          h((T *)c->d_h), A((cpx<T> *)c->d_a), B((cpx<T> *)c->d_b),
+          Kh((const cpx<T> *)c->d_khat) {}
+};
+
+// m -> A (x pass of every instance with demag)
+template <typename T>
+int pass_xfwd(mmc_ctx *c)
 {
-    if (!c->any_demag) return MMC_OK;
-    if (!c->tensor_ready) return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
-    hipStream_t st = c->stream;
-    const cpx<T> *tw = (const cpx<T> *)(c->tsize == 4 ? c->d_tw32 : c->d_tw64);
-    T *m = (T *)c->d_m;
-    T *h = (T *)c->d_h;
-    cpx<T> *A = (cpx<T> *)c->d_a;
-    cpx<T> *B = (cpx<T> *)c->d_b;
-    const cpx<T> *Kh = (const cpx<T> *)c->d_khat;
+    const Bufs<T> b(c);
     for (const Segment &sg : c->xl.seg)
         HIPCHK(c, timed(c, K_XFWD, [&] {
-                   return launch_xfwd<T>(st, c->dInst, c->xl.dev + sg.begin, sg.count, 0, sg.fixed_n,
-                                         sg.rows, m, A, tw, sg.lds);
-               }, sg.bytes));
-    if (c->zl.n())
-        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(st, c->dInst, c->zl.dev, c->zl.n(), 0, A, B, tw, c->zl.lds); }));
-    for (const Segment &sg : c->yl.seg)
-        HIPCHK(c, timed(c, K_YMUL, [&] {
-                   return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, B, Kh,
-                                         tw, sg.lds);
-               }, sg.bytes));
-    if (c->zl.n())
-        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, c->zl.dev, c->zl.n(), B, A, tw, c->zl.lds); }));
-    for (const Segment &sg : c->xl.seg)
-        HIPCHK(c, timed(c, K_XINV, [&] {
-                   return launch_xinv<T>(st, c->dInst, c->xl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
-                                         A, h, tw, sg.lds);
+                   return launch_xfwd<T>(c->stream, c->dInst, c->xl.dev + sg.begin, sg.count, 0,
+                                         sg.fixed_n, sg.rows, b.m, b.A, b.tw, sg.lds);
                }, sg.bytes));
     return MMC_OK;
 }
 
+// A -> (z) -> y, Khat product, y -> (z) -> A
+template <typename T>
+int pass_zyz(mmc_ctx *c)
+{
+    const Bufs<T> b(c);
+    hipStream_t st = c->stream;
+    if (c->zl.n())
+        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(st, c->dInst, c->zl.dev, c->zl.n(), 0, b.A, b.B, b.tw, c->zl.lds); }));
+    for (const Segment &sg : c->yl.seg)
+        HIPCHK(c, timed(c, K_YMUL, [&] {
+                   return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
+                                         b.B, b.Kh, b.tw, sg.lds);
+               }, sg.bytes));
+    if (c->zl.n())
+        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, c->zl.dev, c->zl.n(), b.B, b.A, b.tw, c->zl.lds); }));
+    return MMC_OK;
+}
+
+// Demag field of the current state: m -> h (every instance with demag).
+template <typename T>
+int demag_chain(mmc_ctx *c)
+{
+    if (!c->any_demag) return MMC_OK;
+    if (!c->tensor_ready) return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
+    int rc = pass_xfwd<T>(c);
+    if (rc) return rc;
+    rc = pass_zyz<T>(c);
+    if (rc) return rc;
+    const Bufs<T> b(c);
+    for (const Segment &sg : c->xl.seg)
+        HIPCHK(c, timed(c, K_XINV, [&] {
+                   return launch_xinv<T>(c->stream, c->dInst, c->xl.dev + sg.begin, sg.count, sg.fixed_n,
+                                         sg.rows, b.A, b.h, b.tw, sg.lds);
+               }, sg.bytes));
+    return MMC_OK;
+}
+
+bool fused_sweeps(const mmc_ctx *c) { return c->fuse && !c->metro_tiled; }
+
+// One sweep.  Fused mode expects A to hold the x spectrum of the current
+// state (run_sweeps launches pass_xfwd first) and leaves it holding the
+// spectrum of the new state: z/y/z passes, then inverse x + colour 0, then
+// colour 1 + forward x.
 template <typename T>
 int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
 {
+    hipStream_t st = c->stream;
+    if (fused_sweeps(c)) {
+        int rc = pass_zyz<T>(c);
+        if (rc) return rc;
+        const Bufs<T> b(c);
+        for (const Segment &sg : c->fl.seg)
+            HIPCHK(c, timed(c, K_XINVM, [&] {
+                       return launch_xinv_metro<T>(st, c->dInst, c->fl.dev + sg.begin, sg.count,
+                                                   sg.fixed_n, sg.rows, b.A, b.m, b.h, c->d_beta,
+                                                   c->d_ctl, sweep_off, b.tw, sg.lds);
+                   }, sg.bytes));
+        for (const Segment &sg : c->fl.seg)
+            HIPCHK(c, timed(c, K_MXFWD, [&] {
+                       return launch_metro_xfwd<T>(st, c->dInst, c->fl.dev + sg.begin, sg.count,
+                                                   sg.fixed_n, sg.rows, b.m, b.h, b.A, c->d_beta,
+                                                   c->d_ctl, sweep_off, b.tw, sg.lds);
+                   }, sg.bytes));
+        return MMC_OK;
+    }
     int rc = demag_chain<T>(c);
     if (rc) return rc;
-    hipStream_t st = c->stream;
     if (c->metro_tiled) {
         // one launch: m read, h read and m written once per sweep
         double bytes = 0;
@@ -749,6 +866,10 @@ int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
     hipStream_t st = c->stream;
     if (c->any_demag && !c->tensor_ready)
         return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
+    if (count && fused_sweeps(c)) {
+        int rc = pass_xfwd<T>(c);
+        if (rc) return rc;
+    }
     if (!c->use_graph || c->timing) {
         HIPCHK(c, launch_set_ctl(st, c->d_ctl, stage, sweep0));
         for (uint32_t s = 0; s < count; ++s) {
@@ -764,7 +885,7 @@ int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
         for (int s = 0; s < G && rc == MMC_OK; ++s) rc = enqueue_sweep<T>(c, (uint32_t)s);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
-        if (rc) { if (g) hipGraphDestroy(g); return rc; }
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         HIPCHK(c, e);
         c->graph = g;
         HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
@@ -874,7 +995,7 @@ int tensor_fft(mmc_ctx *c)
                                   c->tyl.lds);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = fail(c, MMC_EHIP, "tensor FFT failed: %s", hipGetErrorString(e));
-    hipFree(KA);
+    (void)hipFree(KA);
     return rc;
 }
 
@@ -946,17 +1067,17 @@ int mmc_batch_create(const mmc_params *p, int count, int precision, mmc_ctx **ou
 void mmc_destroy(mmc_ctx *c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    for (auto e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (void *p : {c->d_m, c->d_h, c->d_a, c->d_khat, (void *)c->d_K, c->d_tw32, c->d_tw64,
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
-        if (p) hipFree(p);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl})
-        if (l->dev) hipFree(l->dev);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl})
+        if (l->dev) (void)hipFree(l->dev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -1021,7 +1142,7 @@ int mmc_build_tensor(mmc_ctx *c, const char *cache_dir)
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) rc = fail(c, MMC_EHIP, "tensor build failed: %s", hipGetErrorString(e));
         }
-        hipFree(dW);
+        (void)hipFree(dW);
         if (rc) return rc;
     }
     if (cache_dir && *cache_dir) {
@@ -1275,9 +1396,25 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
     }
     if (k == "graph_sweeps") {
         if (value < 1 || value > 4096) return fail(c, MMC_EINVAL, "graph_sweeps must be in [1,4096]");
-        if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-        if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
         c->graph_sweeps = (int)value;
+        return MMC_OK;
+    }
+    if (k == "fuse_rows") {
+        // row pairs per fused tile: 1, or (value > 1) the full tile
+        if (value < 1) return fail(c, MMC_EINVAL, "fuse_rows must be >= 1");
+        for (auto &in : c->inst)
+            if (in.tyf) in.tyf = value > 1 ? fused_full(c, in) : 1;
+        return retile(c, 0, 0);
+    }
+    if (k == "fuse") {
+        if (value && !c->fuse_ok)
+            return fail(c, MMC_EUNSUP, "fused x-pass + Metropolis sweeps need demag, a "
+                                       "checkerboard and a fixed-plan x length for every instance");
+        c->fuse = value != 0;
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
         return MMC_OK;
     }
     if (k == "metro_tiled") {

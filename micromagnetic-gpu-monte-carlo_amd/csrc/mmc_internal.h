@@ -47,6 +47,7 @@ struct Inst {
                           // pass ping-pongs two LDS buffers of tile size
     int tyx64, tyz64;     // tile shapes of the FP64 tensor transform
     int mtx, mty;         // Metropolis x-y tile (both colours in one pass)
+    int tyf;              // row pairs per fused x-pass + Metropolis tile (0: none)
     long N, NP;
     long m_off, h_off;    // T
     long a_off, b_off;    // cpx<T>
@@ -111,6 +112,25 @@ template <typename T>
 hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_n, int rows, const cpx<T> *spec, T *real,
                        const cpx<T> *tw, size_t lds);
+// Fused x pass + Metropolis tiles: `rows` row pairs of all three components
+// of x length fixed_n (a fixed-plan length); rows is 1 or
+// fused_rows(fused_rows_base<T>(), fixed_n).
+template <typename T> constexpr int fused_rows_base() { return sizeof(T) == 4 ? 2048 : 1024; }
+constexpr int fused_rows(int base, int n) { return base / n > 1 ? base / n : 1; }
+// Inverse x pass + colour 0: reads spec, updates m, stores colour-1 h
+// compactly in the h buffer.
+template <typename T>
+hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n,
+                             int fixed_n, int rows, const cpx<T> *spec, T *m,
+                             T *h, const double *beta, const uint32_t *ctl,
+                             uint32_t sweep_off, const cpx<T> *tw, size_t lds);
+// Colour 1 (compact h) + forward x pass of the updated rows into spec.
+template <typename T>
+hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n,
+                             int fixed_n, int rows, T *m, const T *h,
+                             cpx<T> *spec, const double *beta,
+                             const uint32_t *ctl, uint32_t sweep_off,
+                             const cpx<T> *tw, size_t lds);
 template <typename T>
 hipError_t launch_metro(hipStream_t st, const Inst *I, const Work *W, int n,
                         T *m, const T *h, const double *beta,

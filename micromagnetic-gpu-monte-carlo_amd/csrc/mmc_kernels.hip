@@ -12,6 +12,8 @@
 //   k_yfwd_tensor   y transform of the tensor, scaled by 1/(PxPyPz)
 //   k_metro         Metropolis update of one colour class (mcKernel2,
 //                   src/mc_sampling.h:27-41)
+//   k_xinv_metro    inverse x pass fused with the first checkerboard colour
+//   k_metro_xfwd    second checkerboard colour fused with the forward x pass
 //   k_energy(+final) per-cell energy terms and <m>, compensated FP64
 //                   block partials, fixed-order final sum (src/main.cu:945-947)
 //   k_field         effective field in A/m (effectiveHKernel,
@@ -992,6 +994,199 @@ __global__ __launch_bounds__(kThreads) void k_metro_tile(const Inst *__restrict_
     }
 }
 
+// Metropolis decision for checkerboard cell (x, y, z) = flat index i with
+// demag field hi: neighbours from global m (they are of the other colour, so
+// no workgroup writes them during this pass).  Writes the accepted state to
+// global m and returns the cell's state after the update.
+template <typename T>
+__device__ __forceinline__ V3<T> metro_update(const Inst &in, T *m, int x, int y, int z, long i,
+                                              const V3<T> &mi, const V3<T> &hi, uint32_t sweep,
+                                              uint32_t stage, T bt)
+{
+    V3<T> ex, dmi, mn;
+    T cnt;
+    stencil(in, m, x, y, z, ex, dmi, cnt);
+    if (metro_cell(in, mi, ex, dmi, hi, 1, (uint32_t)i, sweep, stage, bt, mn)) {
+        const long N = in.N;
+        m[i] = mn.x;
+        m[N + i] = mn.y;
+        m[2 * N + i] = mn.z;
+        return mn;
+    }
+    return mi;
+}
+
+// Inverse x pass of all three components of 2R real rows (item (inst, 0, z,
+// y0): rows y0.. of layer z) fused with the first checkerboard colour of
+// those rows.  The h rows stay in LDS for colour 0 (cells with x + y + z
+// even); the h of colour-1 cells is stored compactly for the second pass:
+// hc[comp][(z*ny + y)*hw + j], x = 2j + ((y + z + 1) & 1), hw = ceil(nx/2),
+// in the instance's h region.  Same arithmetic as k_xinv_n + k_metro.
+template <typename T, int N, int R>
+__global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict__ I,
+                                                         const Work *__restrict__ W,
+                                                         const cpx<T> *__restrict__ specb,
+                                                         T *__restrict__ mb, T *__restrict__ hb,
+                                                         const double *__restrict__ beta,
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint32_t sweep_off,
+                                                         const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);    // row comp*R + pair
+    cpx<T> *tw = s + 3 * R * N;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int z = w.b, y0 = w.c;
+    constexpr int H = N / 2 + 1;
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const int nyt = min(2 * R, ny - y0);
+    const int npair = (nyt + 1) >> 1;
+    const long cstride = (long)H * nz * ny;
+    const cpx<T> *src = specb + in.a_off;
+    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
+    if (npair < R) {
+        const int zr = (R - npair) * N;
+        for (int e = threadIdx.x; e < 3 * zr; e += kThreads) {
+            const int c = e / zr, r = e - c * zr;
+            s[(c * R + npair) * N + r] = {T(0), T(0)};
+        }
+    }
+    const int per = H * npair;
+    for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+        const int c = e / per, r = e - c * per;
+        const int k = r / npair, q = r - k * npair;
+        const cpx<T> *p = src + c * cstride + ((long)k * nz + z) * ny + y0 + 2 * q;
+        const cpx<T> a = p[0];
+        T bx = T(0), by = T(0);
+        if (2 * q + 1 < nyt) {
+            const cpx<T> b = p[1];
+            bx = b.x;
+            by = b.y;
+        }
+        cpx<T> *row = s + (c * R + q) * N;
+        row[k] = {a.x - by, -(a.y + bx)};
+        if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
+    }
+    __syncthreads();
+    fft_fixed<T, N, 3 * R>(s, tw);
+    auto hval = [&](int yy, int x) -> V3<T> {
+        const int o = (yy >> 1) * N + x;
+        if (yy & 1) return {-s[o].y, -s[R * N + o].y, -s[2 * R * N + o].y};
+        return {s[o].x, s[R * N + o].x, s[2 * R * N + o].x};
+    };
+    T *m = mb + in.m_off;
+    T *hc = hb + in.h_off;
+    const long NN = in.N;
+    const int hw = (nx + 1) >> 1;
+    const long cst = (long)hw * ny * nz;
+    const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
+    const T bt = T(beta[w.inst]);
+    for (int e = threadIdx.x; e < nyt * hw; e += kThreads) {
+        const int yy = e / hw, j = e - yy * hw, y = y0 + yy;
+        const int p0 = (y + z) & 1;
+        const int xa = 2 * j + p0, xb = 2 * j + (p0 ^ 1);
+        const long rowi = (long)nx * (y + (long)ny * z);
+        if (xb < nx) {
+            const V3<T> h1 = hval(yy, xb);
+            const long o = ((long)z * ny + y) * hw + j;
+            hc[o] = h1.x;
+            hc[cst + o] = h1.y;
+            hc[2 * cst + o] = h1.z;
+        }
+        if (xa < nx) {
+            const long i = rowi + xa;
+            const V3<T> mi = load_m(m, NN, i);
+            metro_update(in, m, xa, y, z, i, mi, hval(yy, xa), sweep, stage, bt);
+        }
+    }
+}
+
+// Second checkerboard colour of 2R real rows fused with the forward x pass
+// of the rows' final state (item as k_xinv_metro).  The rows are loaded into
+// LDS packed two real rows per complex row, colour-1 cells are updated from
+// the compact h and written to global m and to LDS, then the rows are
+// transformed and their half spectrum stored as k_xfwd_n does.
+template <typename T, int N, int R>
+__global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict__ I,
+                                                         const Work *__restrict__ W,
+                                                         T *__restrict__ mb,
+                                                         const T *__restrict__ hb,
+                                                         cpx<T> *__restrict__ specb,
+                                                         const double *__restrict__ beta,
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint32_t sweep_off,
+                                                         const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + 3 * R * N;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int z = w.b, y0 = w.c;
+    constexpr int H = N / 2 + 1;
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const int nyt = min(2 * R, ny - y0);
+    const long NN = in.N;
+    T *m = mb + in.m_off;
+    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
+    for (int e = threadIdx.x; e < 3 * R * N; e += kThreads) {
+        const int c = e / (R * N), r = e - c * (R * N);
+        const int q = r / N, x = r - q * N;
+        T re = T(0), im = T(0);
+        if (x < nx && 2 * q < nyt) {
+            const T *row = m + c * NN + (long)nx * ((y0 + 2 * q) + (long)ny * z);
+            re = row[x];
+            if (2 * q + 1 < nyt) im = row[(long)nx + x];
+        }
+        s[e] = {re, im};
+    }
+    __syncthreads();
+    {
+        const T *hc = hb + in.h_off;
+        const int hw = (nx + 1) >> 1;
+        const long cst = (long)hw * ny * nz;
+        const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
+        const T bt = T(beta[w.inst]);
+        for (int e = threadIdx.x; e < nyt * hw; e += kThreads) {
+            const int yy = e / hw, j = e - yy * hw, y = y0 + yy;
+            const int x = 2 * j + ((y + z + 1) & 1);
+            if (x >= nx) continue;
+            const int o = (yy >> 1) * N + x;
+            const bool odd = yy & 1;
+            const V3<T> mi = odd ? V3<T>{s[o].y, s[R * N + o].y, s[2 * R * N + o].y}
+                                 : V3<T>{s[o].x, s[R * N + o].x, s[2 * R * N + o].x};
+            const long oh = ((long)z * ny + y) * hw + j;
+            const V3<T> hi = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
+            const long i = (long)nx * (y + (long)ny * z) + x;
+            const V3<T> mn = metro_update(in, m, x, y, z, i, mi, hi, sweep, stage, bt);
+            if (odd) {
+                s[o].y = mn.x;
+                s[R * N + o].y = mn.y;
+                s[2 * R * N + o].y = mn.z;
+            } else {
+                s[o].x = mn.x;
+                s[R * N + o].x = mn.y;
+                s[2 * R * N + o].x = mn.z;
+            }
+        }
+    }
+    __syncthreads();
+    fft_fixed<T, N, 3 * R>(s, tw);
+    const int npair = (nyt + 1) >> 1;
+    const long cstride = (long)H * nz * ny;
+    cpx<T> *dst = specb + in.a_off;
+    const int per = H * npair;
+    for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+        const int c = e / per, r = e - c * per;
+        const int k = r / npair, q = r - k * npair;
+        const cpx<T> *row = s + (c * R + q) * N;
+        const cpx<T> A = row[k];
+        const cpx<T> B = row[k ? N - k : 0];
+        cpx<T> *o = dst + c * cstride + ((long)k * nz + z) * ny + y0 + 2 * q;
+        o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+        if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+    }
+}
+
 // ============================================================================
 // (e) energy terms and <m>, compensated FP64 reductions
 // ============================================================================
@@ -1209,6 +1404,12 @@ hipError_t launch_tensor(hipStream_t st, const Inst *I, const Work *W, int n, do
 // Fixed x-pass variants: sizes x {1 row pair, kXRows<T> row pairs}.
 #define MMC_XFIXED(M, T, R) M(T, 128, R) M(T, 192, R) M(T, 256, R) M(T, 384, R) \
     M(T, 512, R) M(T, 1024, R) M(T, 2048, R)
+// fused x + Metropolis variants: every fixed x length, one row pair or the
+// full tile fused_rows(F, N)
+#define MMC_FUSED(M, T, F) M(T, 128, 1) M(T, 192, 1) M(T, 256, 1) M(T, 384, 1) M(T, 512, 1)   \
+    M(T, 1024, 1) M(T, 2048, 1) M(T, 128, fused_rows(F, 128)) M(T, 192, fused_rows(F, 192))    \
+    M(T, 256, fused_rows(F, 256)) M(T, 384, fused_rows(F, 384)) M(T, 512, fused_rows(F, 512))  \
+    M(T, 1024, fused_rows(F, 1024))
 
 template <typename T>
 hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode, int fixed_n,
@@ -1285,6 +1486,44 @@ hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n, int 
 #undef MMC_XI_CASE
     MMC_LAUNCH(k_xinv<T>, g, b, lds, st, I, W, spec, real, tw);
     return post();
+}
+
+template <typename T>
+hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
+                             int rows, const cpx<T> *spec, T *m, T *h, const double *beta,
+                             const uint32_t *ctl, uint32_t sweep_off, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 g(n), b(kThreads);
+    constexpr int F = fused_rows_base<T>();
+#define MMC_XM_CASE(TT, NN, RR)                                                                  \
+    if (fixed_n == NN && rows == RR) {                                                           \
+        MMC_LAUNCH((k_xinv_metro<TT, NN, RR>), g, b, lds, st, I, W, spec, m, h, beta, ctl,       \
+                   sweep_off, tw);                                                               \
+        return post();                                                                           \
+    }
+    MMC_FUSED(MMC_XM_CASE, T, F)
+#undef MMC_XM_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
+                             int rows, T *m, const T *h, cpx<T> *spec, const double *beta,
+                             const uint32_t *ctl, uint32_t sweep_off, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 g(n), b(kThreads);
+    constexpr int F = fused_rows_base<T>();
+#define MMC_MX_CASE(TT, NN, RR)                                                                  \
+    if (fixed_n == NN && rows == RR) {                                                           \
+        MMC_LAUNCH((k_metro_xfwd<TT, NN, RR>), g, b, lds, st, I, W, m, h, spec, beta, ctl,       \
+                   sweep_off, tw);                                                               \
+        return post();                                                                           \
+    }
+    MMC_FUSED(MMC_MX_CASE, T, F)
+#undef MMC_MX_CASE
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -1389,6 +1628,10 @@ static hipError_t set_lds_t(int bytes)
     one((const void *)k_ymul_n<T, 2048, fixed_cols(2048)>);
     one((const void *)k_yfwd_tensor<T>);
     one((const void *)k_metro_tile<T>);
+    constexpr int F = fused_rows_base<T>();
+#define MMC_FL(TT, NN, RR) one((const void *)k_xinv_metro<TT, NN, RR>); one((const void *)k_metro_xfwd<TT, NN, RR>);
+    MMC_FUSED(MMC_FL, T, F)
+#undef MMC_FL
     return e;
 }
 
@@ -1416,6 +1659,14 @@ hipError_t set_lds_limit(size_t bytes)
     template hipError_t launch_metro<T>(hipStream_t, const Inst *, const Work *, int, T *,    \
                                         const T *, const double *, const uint32_t *,          \
                                         uint32_t, int, int);                                  \
+    template hipError_t launch_xinv_metro<T>(hipStream_t, const Inst *, const Work *, int, int, \
+                                             int, const cpx<T> *, T *, T *, const double *,  \
+                                             const uint32_t *, uint32_t, const cpx<T> *,     \
+                                             size_t);                                        \
+    template hipError_t launch_metro_xfwd<T>(hipStream_t, const Inst *, const Work *, int, int, \
+                                             int, T *, const T *, cpx<T> *, const double *,  \
+                                             const uint32_t *, uint32_t, const cpx<T> *,     \
+                                             size_t);                                        \
     template hipError_t launch_metro_tile<T>(hipStream_t, const Inst *, const Work *, int,    \
                                              T *, const T *, const double *, const uint32_t *, \
                                              uint32_t, int, size_t);                          \

@@ -97,8 +97,11 @@ def test_init_random_matches_oracle(orc, mmc):
 @pytest.mark.parametrize("geo", [dict(nx=16, ny=16, nz=1),
                                  dict(nx=12, ny=10, nz=1, pbc_x=1, pbc_y=1),
                                  dict(nx=9, ny=8, nz=1, pbc_x=2),
-                                 dict(nx=8, ny=6, nz=3, pbc_z=1)],
-                         ids=["open", "pbc", "odd_pbc_3colors", "3d"])
+                                 dict(nx=8, ny=6, nz=3, pbc_z=1),
+                                 dict(nx=64, ny=9, nz=1),
+                                 dict(nx=64, ny=6, nz=2, pbc_y=1)],
+                         ids=["open", "pbc", "odd_pbc_3colors", "3d", "fused_open",
+                              "fused_3d"])
 def test_sweep_trajectory_fp64(orc, mmc, geo):
     """Same Philox stream, same colour order, frozen demag: identical
     accept/reject decisions, so m agrees to rounding after several sweeps."""
@@ -131,7 +134,7 @@ def test_determinism_and_batch(orc, mmc):
     """Bit-identical reruns; a batch of uneven instances gives each
     instance exactly what its own single context gives."""
     geos = [film(nx=16, ny=16, nz=1, seed=1), film(nx=24, ny=20, nz=1, pbc_x=1, seed=2),
-            film(nx=12, ny=12, nz=2, bext_z=0.5, seed=3)]
+            film(nx=12, ny=12, nz=2, bext_z=0.5, seed=3), film(nx=64, ny=10, nz=1, seed=4)]
     singles = []
     for kw in geos:
         c = mmc.Context(mmc.make_params(**kw), 1)
@@ -152,7 +155,7 @@ def test_determinism_and_batch(orc, mmc):
     b2.init_random_m()
     b2.set_temperature(500.0)
     b2.sweeps(0, 0, 5)
-    for i in range(3):
+    for i in range(len(geos)):
         assert np.array_equal(b2.m(i), b.m(i))
 
 
@@ -220,3 +223,38 @@ def test_tiled_metropolis_matches_per_colour(mmc, geo, precision):
         c.sweeps(3, 0, 7)
         out.append(c.m())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("geo", [dict(nx=64, ny=40, nz=1),
+                                 dict(nx=128, ny=64, nz=1, pbc_x=1, pbc_y=2),
+                                 dict(nx=91, ny=37, nz=1),
+                                 dict(nx=64, ny=24, nz=3),
+                                 dict(nx=64, ny=16, nz=4, pbc_z=1),
+                                 dict(nx=256, ny=50, nz=1, pbc_x=3)],
+                         ids=["open", "pbc", "odd_open", "3d_open", "3d_pbcz", "wide_pbc"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_fused_sweeps_match_separate(mmc, geo, precision):
+    """Inverse x pass fused with colour 0 and colour 1 fused with the next
+    forward x pass (one row pair or full tiles per workgroup, eager and
+    graph-replayed) give the bit-identical chain and energies of separate
+    launches."""
+    kw = film(**geo)
+    out = []
+    for fuse, rows, graph in ((0, 1, 0), (1, 1, 0), (1, 2, 0), (1, 2, 1)):
+        c = mmc.Context(mmc.make_params(**kw), precision)
+        c.build_tensor()
+        c.init_random_m()
+        c.set_temperature(650.0)
+        c.set_option("fuse", fuse)
+        c.set_option("fuse_rows", rows)
+        if graph:
+            c.set_option("graph", 1)
+            c.set_option("graph_sweeps", 3)
+        c.sweeps(2, 0, 4)
+        m1 = c.m()
+        c.sweeps(2, 4, 5)
+        out.append((m1, c.m(), c.energy()["total"]))
+    for m1, m2, e in out[1:]:
+        assert np.array_equal(m1, out[0][0])
+        assert np.array_equal(m2, out[0][1])
+        assert e == out[0][2]
