@@ -90,7 +90,15 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
     c.sweeps(0, warmup, steps)
     c.sync()
     dt = time.perf_counter() - t0
-    ksteps = ksteps or min(steps, 400)
+    if ksteps is None:
+        ksteps = min(steps, 400)
+    kern = {}
+    dt_eager = None
+    if ksteps <= 0:
+        E = c.energy()
+        return dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
+                    eager_timed_attempts_per_s=None, tensor_build_s=t_tensor, kernels=kern,
+                    energy_J=E["total"], ctx=c, cells=N)
     c.set_option("timing", 1)
     for k in mod.KINDS:
         c.kernel_stats(k, reset=True)
@@ -98,7 +106,6 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
     c.sweeps(0, warmup + steps, ksteps)
     c.sync()
     dt_eager = time.perf_counter() - t1
-    kern = {}
     for k in mod.KINDS:
         n, ms, by = c.kernel_stats(k, reset=True)
         if n:
@@ -131,6 +138,8 @@ def batch_params(count=64, master_seed=2024):
 def roofline(kern):
     """Dominant kernel (largest device time) and its achieved bandwidth:
     algorithmic bytes of its timed launches over their summed duration."""
+    if not kern:
+        return None, None
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     k = kern[dom]
     gbs = k["total_bytes"] / (k["total_ms"] * 1e-3) / 1e9
@@ -168,16 +177,35 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the FP64 and 1024x1024x2 side measurements")
+    ap.add_argument("--workload", choices=("film256", "film1024x2"), default="film256",
+                    help="film measured as `value` (film1024x2 only for profiling runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the timed sweeps eagerly instead of replaying hipGraphs")
+    ap.add_argument("--ksteps", type=int, default=None,
+                    help="event-timed eager sweeps after the timed region (default "
+                         "min(steps, 400); 0 skips them, as hardware-counter runs need)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device to device three times (known-size kernel "
+                         "for scaling hardware byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     mod = load_pkg()
     mod.lib()
+    if a.copy_calibration:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration: 3 device copies of 268435456 bytes")
+        del src, dst
 
-    kw = film256()
-    log(f"[bench] 256x256x1 PBC20 FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
-    r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup)
+    kw = film256() if a.workload == "film256" else film1024x2()
+    log(f"[bench] {a.workload} FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
+    r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps)
     dom, roof = roofline(r32["kernels"])
     extra = {}
     if not a.no_extra:
@@ -230,8 +258,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "anneal sweep, 256x256x1 film, PBC 20x20 images, "
-                               "FP32-bulk, T=1400 K",
+        "config": {"workload": (f"anneal sweep, {kw['nx']}x{kw['ny']}x{kw['nz']} film, "
+                                f"PBC {kw['pbc_x']}x{kw['pbc_y']} images, FP32-bulk, T=1400 K"),
                    "cells": kw["nx"] * kw["ny"] * kw["nz"], "step": "one sweep",
                    "precision": "fp32-bulk"},
         "roofline": roof,

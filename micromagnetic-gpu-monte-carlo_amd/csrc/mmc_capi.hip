@@ -300,6 +300,7 @@ int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
         if (in.pbc[a] && n >= 3 && (n & 1)) in.ncolors = 3;
     }
     in.cal_demag = p.cal_demag ? 1 : 0;
+    for (int c6 = 0; c6 < 6; ++c6) in.kmode[c6] = 2;   // until the tensor is built
     choose_mtile(c, in);
     // physics in reduced units (fields per Ms, energies per mu0 Ms^2 V)
     const double mu0 = 4e-7 * kPiTrunc;          // src/main.cu:244
@@ -338,7 +339,8 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
     const double N = (double)in.N;
     const double spec_x = 3.0 * in.Hx * in.nz * in.ny * cb;   // A
     const double spec_z = 3.0 * in.Hx * in.Pz * in.ny * cb;   // B
-    const double khat = 6.0 * in.Hx * in.Pz * in.Py * cb;
+    double khat = 0;   // zero planes are not read, real planes are half size
+    for (int c6 = 0; c6 < 6; ++c6) khat += in.kmode[c6] * (double)in.Hx * in.Pz * in.Py * b;
     switch (k) {
     case K_XFWD: case K_XINV: return in.cal_demag ? 3 * N * b + spec_x : 0.0;
     case K_ZFWD: case K_ZINV: return (in.cal_demag && in.Pz > 1) ? spec_x + spec_z : 0.0;
@@ -977,26 +979,75 @@ bool save_cache(const std::string &path, const Inst &in, const mmc_params &p, co
     return (bool)f;
 }
 
-// Tensor FFT (FP64) for all instances with demag: K -> KA -> Khat.
+// FP32 bulk mode: Khat planes whose imaginary part (or whole value) is below
+// this fraction of the instance's largest |Khat| are stored real (or not at
+// all).  The exact tensor of an open axis is circularly even/odd, so its
+// spectrum is real; the Newell cancellation noise of distant offsets leaves
+// imaginary parts of 1e-10..1e-8 of the largest value on open grids up to a
+// few hundred cells (periodic image sums leave far more, and keep complex
+// planes).  Dropping parts below 1e-8 changes fields by less than FP32
+// rounding.  FP64 mode drops only exact zeros.
+constexpr double kKhatDropTolFp32 = 1e-8;
+
+// Tensor FFT (FP64) for all instances with demag: K -> KA -> FP64 Khat,
+// then each component plane is classified zero / real / complex and stored
+// in the bulk precision (round to nearest) in that form.
 template <typename T>
 int tensor_fft(mmc_ctx *c)
 {
     hipStream_t st = c->stream;
-    cpx<double> *KA = nullptr;
+    cpx<double> *KA = nullptr, *KH = nullptr;
     HIPCHK(c, hipMalloc(&KA, std::max<size_t>(1, c->ka_elems) * sizeof(cpx<double>)));
+    hipError_t e = hipMalloc(&KH, std::max<size_t>(1, c->khat_elems) * sizeof(cpx<double>));
+    if (e != hipSuccess) {
+        (void)hipFree(KA);
+        return fail(c, MMC_EHIP, "tensor FFT: %s", hipGetErrorString(e));
+    }
     const cpx<double> *tw = (const cpx<double> *)c->d_tw64;
     int rc = MMC_OK;
-    hipError_t e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, 0, 0, c->d_K, KA, tw,
-                                       c->txl.lds);
+    e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, 0, 0, c->d_K, KA, tw, c->txl.lds);
     if (e == hipSuccess && c->tzl.n())
         e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, KA, KA, tw, c->tzl.lds);
     if (e == hipSuccess)
-        e = launch_yfwd_tensor<T>(st, c->dInst, c->tyl.dev, c->tyl.n(), KA, (cpx<T> *)c->d_khat, tw,
-                                  c->tyl.lds);
+        e = launch_yfwd_tensor<double>(st, c->dInst, c->tyl.dev, c->tyl.n(), KA, KH, tw, c->tyl.lds);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) rc = fail(c, MMC_EHIP, "tensor FFT failed: %s", hipGetErrorString(e));
+    std::vector<cpx<double>> h64;
+    if (e == hipSuccess) {
+        h64.resize(c->khat_elems);
+        e = hipMemcpy(h64.data(), KH, c->khat_elems * sizeof(cpx<double>), hipMemcpyDeviceToHost);
+    }
     (void)hipFree(KA);
-    return rc;
+    (void)hipFree(KH);
+    if (e != hipSuccess) return fail(c, MMC_EHIP, "tensor FFT failed: %s", hipGetErrorString(e));
+    std::vector<T> packed(2 * c->khat_elems, T(0));
+    for (auto &in : c->inst) {
+        if (!in.cal_demag) continue;
+        const long S = (long)in.Hx * in.Pz * in.Py;
+        double mre[6] = {0}, mim[6] = {0}, M = 0;
+        for (int c6 = 0; c6 < 6; ++c6) {
+            const cpx<double> *p = h64.data() + in.khat_off + c6 * S;
+            for (long i = 0; i < S; ++i) {
+                mre[c6] = std::max(mre[c6], std::fabs(p[i].x));
+                mim[c6] = std::max(mim[c6], std::fabs(p[i].y));
+            }
+            M = std::max(M, std::max(mre[c6], mim[c6]));
+        }
+        const double lim = (c->tsize == 4 ? kKhatDropTolFp32 : 0.0) * M;
+        for (int c6 = 0; c6 < 6; ++c6) {
+            in.kmode[c6] = std::max(mre[c6], mim[c6]) <= lim ? 0 : (mim[c6] <= lim ? 1 : 2);
+            const cpx<double> *p = h64.data() + in.khat_off + c6 * S;
+            T *o = packed.data() + 2 * (in.khat_off + c6 * S);
+            if (in.kmode[c6] == 1)
+                for (long i = 0; i < S; ++i) o[i] = (T)p[i].x;
+            else if (in.kmode[c6] == 2)
+                for (long i = 0; i < S; ++i) { o[2 * i] = (T)p[i].x; o[2 * i + 1] = (T)p[i].y; }
+        }
+    }
+    if (c->khat_elems)
+        HIPCHK(c, hipMemcpy(c->d_khat, packed.data(), packed.size() * sizeof(T), hipMemcpyHostToDevice));
+    // new plane modes: device copy of the instances, per-launch bytes
+    const int r2 = retile(c, 0, 0);
+    return r2 ? r2 : rc;
 }
 
 int refresh_mirror(mmc_ctx *c)

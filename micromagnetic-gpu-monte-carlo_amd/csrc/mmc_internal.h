@@ -48,6 +48,8 @@ struct Inst {
     int tyx64, tyz64;     // tile shapes of the FP64 tensor transform
     int mtx, mty;         // Metropolis x-y tile (both colours in one pass)
     int tyf;              // row pairs per fused x-pass + Metropolis tile (0: none)
+    int kmode[6];         // Khat storage of each component: 0 zero, 1 real
+                          // (imaginary part below rounding), 2 complex
     long N, NP;
     long m_off, h_off;    // T
     long a_off, b_off;    // cpx<T>

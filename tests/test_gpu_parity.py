@@ -63,10 +63,13 @@ def test_demag_impulse_response(orc, mmc):
     c.set_m(m)
     H = c.field(16) / kw["ms"]
     K = o.tensor()
-    # offsets (x, y) >= 0 sit at padded index (x, y)
+    # offsets (x, y) >= 0 sit at padded index (x, y); the xz entries of a
+    # one-layer film are zero up to rounding, so errors are measured against
+    # the column's scale
+    scale = max(np.abs(K[k, 0, :12, :20]).max() for k in (0, 1, 2))
     for comp, kidx in ((0, 0), (1, 1), (2, 2)):
         ref = K[kidx, 0, :12, :20]
-        assert rel_inf(H[comp, 0], ref) <= 1e-12
+        assert np.abs(H[comp, 0] - ref).max() <= 1e-12 * scale
 
 
 def test_tensor_gpu_vs_oracle(orc, mmc):
@@ -258,3 +261,23 @@ def test_fused_sweeps_match_separate(mmc, geo, precision):
         assert np.array_equal(m1, out[0][0])
         assert np.array_equal(m2, out[0][1])
         assert e == out[0][2]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_khat_planes_stored_compactly(mmc, precision):
+    """FP32 bulk: the Khat planes of an open film are real to far below FP32
+    resolution and its xz, yz planes vanish, so the y pass reads four real
+    planes; an arbitrary tensor keeps six complex planes.  FP64 keeps every
+    plane complex (the imaginary parts left by the Newell cancellation
+    noise, ~1e-10 of the largest value, matter at 1e-12)."""
+    b = 8 if precision == 0 else 4
+    c = mmc.Context(mmc.make_params(**film(nx=64, ny=64, nz=1)), precision)
+    c.build_tensor()
+    Px, Py, Pz = c.P()
+    H = Px // 2 + 1
+    spec = 2 * 3 * H * Pz * c.n()[1] * 2 * b
+    plane = H * Pz * Py * b
+    assert c.kernel_bytes("ymul") == spec + (4 if precision == 1 else 12) * plane
+    K = np.random.default_rng(7).standard_normal(c.tensor().shape)
+    c.set_tensor(K)
+    assert c.kernel_bytes("ymul") == spec + 12 * plane
