@@ -73,7 +73,8 @@ def make_ctx(mod, kws, precision, T=1400.0):
     return c, t_tensor
 
 
-def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=None):
+def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=None,
+            default_stream=False):
     """Warm up, time `steps` sweeps (hipGraph replay when `graph`), then time
     `ksteps` more sweeps with every launch bracketed by dispatch-packet HIP
     events for the per-kernel durations."""
@@ -81,6 +82,9 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
         kws = [kws]
     c, t_tensor = make_ctx(mod, kws, precision, T)
     N = sum(kw["nx"] * kw["ny"] * kw["nz"] for kw in kws)
+    if default_stream:
+        c.set_option("default_stream", 1)
+        graph = False
     if graph:
         c.set_option("graph", 1)
         c.set_option("graph_sweeps", 20)
@@ -181,6 +185,9 @@ def main():
                     help="film measured as `value` (film1024x2 only for profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the timed sweeps eagerly instead of replaying hipGraphs")
+    ap.add_argument("--default-stream", action="store_true",
+                    help="launch on the device's default stream (hardware-counter runs; "
+                         "implies --no-graph)")
     ap.add_argument("--ksteps", type=int, default=None,
                     help="event-timed eager sweeps after the timed region (default "
                          "min(steps, 400); 0 skips them, as hardware-counter runs need)")
@@ -205,7 +212,8 @@ def main():
 
     kw = film256() if a.workload == "film256" else film1024x2()
     log(f"[bench] {a.workload} FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
-    r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps)
+    r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps,
+                  default_stream=a.default_stream)
     dom, roof = roofline(r32["kernels"])
     extra = {}
     if not a.no_extra:

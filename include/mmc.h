@@ -135,6 +135,14 @@ int mmc_field(mmc_ctx *ctx, int inst, int terms, double *hx, double *hy,
 int mmc_energy(mmc_ctx *ctx, int inst, double *E_total, double e_terms[5],
                double m_avg[3]);
 
+/* Energy change [J] of every cell's Metropolis trial move for (stage,
+ * sweep) — Philox counter (cell, sweep, stage, 0) — against the current
+ * state and its demag field, the moves not applied (N doubles).  The dE
+ * that mcKernel2 computes internally (src/mc_sampling.h:27-41); exposed to
+ * state the FP32 error of dE against FP64. */
+int mmc_trial_delta_e(mmc_ctx *ctx, int inst, uint32_t stage, uint32_t sweep,
+                      double *dE);
+
 /* ---- Monte Carlo ------------------------------------------------------ */
 
 /* Set the temperature [K] of one instance (inst >= 0) or all (-1). */
@@ -186,6 +194,8 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * "graph_sweeps" (sweeps per captured graph, default 16),
  * "timing" (1 = time every launch with hipEvents recorded by its dispatch
  * packet on the context's stream; sweeps then launch eagerly),
+ * "default_stream" (1 = launch on the device's default stream instead of the
+ * context's own, for profilers that observe only that one; disables "graph"),
  * "tyx" (x-pass rows per workgroup), "ncol" (y-pass columns per workgroup),
  * "metro_tiled" (1: both checkerboard colours in one tiled launch where
  * every instance has a checkerboard; 0, the default: one launch per colour),

@@ -108,7 +108,8 @@ EXPORTS = (
     "mmc_create", "mmc_batch_create", "mmc_destroy", "mmc_last_error",
     "mmc_dims", "mmc_count", "mmc_build_tensor", "mmc_set_tensor",
     "mmc_get_tensor", "mmc_mirror_tensor", "mmc_set_m", "mmc_get_m",
-    "mmc_init_random_m", "mmc_field", "mmc_energy", "mmc_set_temperature",
+    "mmc_init_random_m", "mmc_field", "mmc_trial_delta_e", "mmc_energy",
+    "mmc_set_temperature",
     "mmc_sweeps", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
     "mmc_kernel_stats", "mmc_kernel_bytes",
     "mmc_ovf_read", "mmc_ovf_write", "mmc_ovf_mesh_equal", "mmc_ini_parse",
@@ -166,6 +167,7 @@ def lib() -> C.CDLL:
     L.mmc_get_m.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mmc_init_random_m.argtypes = [vp, C.c_int]
     L.mmc_field.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
+    L.mmc_trial_delta_e.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, dp]
     L.mmc_energy.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mmc_set_temperature.argtypes = [vp, C.c_int, C.c_double]
     L.mmc_sweeps.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32]
@@ -332,6 +334,14 @@ class Context:
         out = np.empty((3, nz, ny, nx))
         self._check(lib().mmc_field(self._ptr, inst, terms, _dp(out[0]),
                                     _dp(out[1]), _dp(out[2])))
+        return out
+
+    def trial_delta_e(self, stage: int, sweep: int, inst: int = 0) -> np.ndarray:
+        """Energy change [J] of every cell's trial move for (stage, sweep),
+        moves not applied; shape (nz, ny, nx)."""
+        nx, ny, nz = self.n(inst)
+        out = np.empty((nz, ny, nx))
+        self._check(lib().mmc_trial_delta_e(self._ptr, inst, stage, sweep, _dp(out)))
         return out
 
     def energy(self, inst: int = 0) -> dict:

@@ -1320,6 +1320,30 @@ int mmc_field(mmc_ctx *c, int inst, int terms, double *hx, double *hy, double *h
     return MMC_OK;
 }
 
+int mmc_trial_delta_e(mmc_ctx *c, int inst, uint32_t stage, uint32_t sweep, double *dE)
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!dE) return fail(c, MMC_EINVAL, "null output");
+    const Inst &in = c->inst[inst];
+    if (in.cal_demag) {
+        rc = c->tsize == 4 ? demag_chain<float>(c) : demag_chain<double>(c);
+        if (rc) return rc;
+    }
+    const int b0 = c->cl.first[inst], b1 = c->cl.first[inst + 1];
+    hipError_t e = c->tsize == 4
+                       ? launch_delta_e<float>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0,
+                                               (const float *)c->d_m, (const float *)c->d_h,
+                                               stage, sweep, c->d_H)
+                       : launch_delta_e<double>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0,
+                                                (const double *)c->d_m, (const double *)c->d_h,
+                                                stage, sweep, c->d_H);
+    HIPCHK(c, e);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(dE, c->d_H, sizeof(double) * in.N, hipMemcpyDeviceToHost));
+    return MMC_OK;
+}
+
 int mmc_energy(mmc_ctx *c, int inst, double *E_total, double e_terms[5], double m_avg[3])
 {
     int rc = check_inst(c, inst);
@@ -1442,7 +1466,24 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
     if (!c || !key) return MMC_EINVAL;
     std::string k(key);
     if (k == "graph") {
+        if (value && !c->stream)
+            return fail(c, MMC_EUNSUP, "graph replay needs the context's own stream");
         c->use_graph = value != 0;
+        return MMC_OK;
+    }
+    if (k == "default_stream") {
+        // 1: launch on the device's default stream (for tools that only
+        // observe it); 0: back to a stream of the context's own
+        HIPCHK(c, c->stream ? hipStreamSynchronize(c->stream) : hipDeviceSynchronize());
+        if (value && c->stream) {
+            if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+            HIPCHK(c, hipStreamDestroy(c->stream));
+            c->stream = nullptr;
+            c->use_graph = false;
+        } else if (!value && !c->stream) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        }
         return MMC_OK;
     }
     if (k == "graph_sweeps") {

@@ -18,6 +18,7 @@
 //                   block partials, fixed-order final sum (src/main.cu:945-947)
 //   k_field         effective field in A/m (effectiveHKernel,
 //                   src/effective_H.h:25-33)
+//   k_delta_e       energy change of every cell's trial move (diagnostic)
 //   k_init          Philox uniform-on-sphere start state
 #include "fft_fixed.h"
 #include "fft_lds.h"
@@ -781,15 +782,15 @@ __device__ __forceinline__ T quad_k0(const Inst &in, V3<T> a)
            T(2) * (k1 * a.x * a.y + k2 * a.x * a.z + k4 * a.y * a.z);
 }
 
-// Metropolis decision for one cell.  dE (units mu0 Ms^2 V) =
+// Trial move and energy change of one cell.  dE (units mu0 Ms^2 V) =
 //   -d.(ex + dmi + b + h - K0 m) - kap[(m'.u)^2 - (m.u)^2] - [m'K0m' - mK0m]/2
-// accept if dE <= 0 or u < exp(-dE * beta), beta = mu0 Ms^2 V / (kB T).
-// Trial m' uniform on the sphere from Philox counter (cell, sweep, stage, 0).
+// with d = m' - m; trial m' uniform on the sphere from Philox counter
+// (cell, sweep, stage, 0); rr returns the counter's four outputs.
 template <typename T>
-__device__ __forceinline__ bool metro_cell(const Inst &in, const V3<T> &mi, const V3<T> &ex,
-                                           const V3<T> &dmi, const V3<T> &hi, int use_demag,
-                                           uint32_t cell, uint32_t sweep, uint32_t stage, T bt,
-                                           V3<T> &mn)
+__device__ __forceinline__ T trial_de(const Inst &in, const V3<T> &mi, const V3<T> &ex,
+                                      const V3<T> &dmi, const V3<T> &hi, int use_demag,
+                                      uint32_t cell, uint32_t sweep, uint32_t stage, V3<T> &mn,
+                                      uint4 &rr)
 {
     V3<T> g = {ex.x + dmi.x + T(in.b[0]), ex.y + dmi.y + T(in.b[1]), ex.z + dmi.z + T(in.b[2])};
     if (use_demag) {
@@ -799,7 +800,7 @@ __device__ __forceinline__ bool metro_cell(const Inst &in, const V3<T> &mi, cons
         g.y += hi.y - (k1 * mi.x + k3 * mi.y + k4 * mi.z);
         g.z += hi.z - (k2 * mi.x + k4 * mi.y + k5 * mi.z);
     }
-    const uint4 rr = philox10(make_uint4(cell, sweep, stage, 0u), in.key[0], in.key[1]);
+    rr = philox10(make_uint4(cell, sweep, stage, 0u), in.key[0], in.key[1]);
     const T zt = T(1) - T(2) * u01<T>(rr.x);
     const T st = sqrt_t(max(T(0), T(1) - zt * zt));
     T sn, cs;
@@ -812,6 +813,20 @@ __device__ __forceinline__ bool metro_cell(const Inst &in, const V3<T> &mi, cons
     const T ou = mi.x * ux + mi.y * uy + mi.z * uz;
     dE -= T(in.kap) * (pu * pu - ou * ou);
     if (use_demag) dE -= T(0.5) * (quad_k0(in, mn) - quad_k0(in, mi));
+    return dE;
+}
+
+// Metropolis decision for one cell: accept the trial move if dE <= 0 or
+// u < exp(-dE * beta), beta = mu0 Ms^2 V / (kB T), u from the counter's third
+// output.
+template <typename T>
+__device__ __forceinline__ bool metro_cell(const Inst &in, const V3<T> &mi, const V3<T> &ex,
+                                           const V3<T> &dmi, const V3<T> &hi, int use_demag,
+                                           uint32_t cell, uint32_t sweep, uint32_t stage, T bt,
+                                           V3<T> &mn)
+{
+    uint4 rr;
+    const T dE = trial_de(in, mi, ex, dmi, hi, use_demag, cell, sweep, stage, mn, rr);
     return dE <= T(0) || u01<T>(rr.z) < exp_t(-dE * bt);
 }
 
@@ -1356,6 +1371,38 @@ __global__ __launch_bounds__(kThreads) void k_field(const Inst *__restrict__ I,
     H[2 * N + i] = in.ms * double(hz);
 }
 
+// Energy change [J] of every cell's trial move (Philox counter (cell, sweep,
+// stage, 0)) against the current state and its demag field h, the move not
+// applied: the dE that the Metropolis kernels test.  Output N doubles.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_delta_e(const Inst *__restrict__ I,
+                                                      const Work *__restrict__ W,
+                                                      const T *__restrict__ mb,
+                                                      const T *__restrict__ hb, uint32_t stage,
+                                                      uint32_t sweep, double *__restrict__ out)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long i = (long)w.a + threadIdx.x;
+    if (i >= in.N) return;
+    const int nx = in.nx, ny = in.ny;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const T *m = mb + in.m_off;
+    const long N = in.N;
+    const V3<T> mi = load_m(m, N, i);
+    V3<T> ex, dmi, mn;
+    T cnt;
+    stencil(in, m, x, y, z, ex, dmi, cnt);
+    V3<T> hi = {T(0), T(0), T(0)};
+    if (in.cal_demag) hi = load_m(hb + in.h_off, N, i);
+    uint4 rr;
+    const T dE = trial_de(in, mi, ex, dmi, hi, in.cal_demag, (uint32_t)i, sweep, stage, mn, rr);
+    out[i] = in.e_unit * double(dE);
+}
+
 // Uniform-on-sphere start state, Philox counter (cell, 0, 0, 1).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_init(const Inst *__restrict__ I,
@@ -1598,6 +1645,15 @@ hipError_t launch_field(hipStream_t st, const Inst *I, const Work *W, int n, con
 }
 
 template <typename T>
+hipError_t launch_delta_e(hipStream_t st, const Inst *I, const Work *W, int n, const T *m,
+                          const T *h, uint32_t stage, uint32_t sweep, double *out)
+{
+    if (n <= 0) return hipSuccess;
+    MMC_LAUNCH(k_delta_e<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, stage, sweep, out);
+    return post();
+}
+
+template <typename T>
 hipError_t launch_init(hipStream_t st, const Inst *I, const Work *W, int n, T *m)
 {
     if (n <= 0) return hipSuccess;
@@ -1696,6 +1752,8 @@ hipError_t set_lds_limit(size_t bytes)
                                          const T *, const T *, double *, int);                \
     template hipError_t launch_field<T>(hipStream_t, const Inst *, const Work *, int,         \
                                         const T *, const T *, int, double *);                 \
+    template hipError_t launch_delta_e<T>(hipStream_t, const Inst *, const Work *, int,       \
+                                          const T *, const T *, uint32_t, uint32_t, double *); \
     template hipError_t launch_init<T>(hipStream_t, const Inst *, const Work *, int, T *);
 
 MMC_INST(float)

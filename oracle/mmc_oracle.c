@@ -752,6 +752,27 @@ int orc_cell_color(const orc_ctx *c, int x, int y, int z)
  * :1101), then every colour class in order.  Trial direction uniform on the
  * sphere, accept if dE <= 0 or u < exp(-dE / kB T).  Philox key = seed,
  * counter = (cell, sweep, stage, 0).  Returns accepted moves. */
+/* Energy change of every cell's trial move for (stage, sweep) against the
+ * current state, moves not applied: the dE that orc_sweep tests (the
+ * quantity mcKernel2 forms from its inputs, src/mc_sampling.h:27-41). */
+void orc_trial_delta_e(orc_ctx *c, uint32_t stage, uint32_t sweep, double *dE)
+{
+    const long N = c->N;
+    uint32_t key[2] = {(uint32_t)c->p.seed, (uint32_t)(c->p.seed >> 32)};
+    if (c->p.cal_demag) orc_demag_fft(c, c->h);
+    else memset(c->h, 0, sizeof(double) * 3 * N);
+    #pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        uint32_t ctr[4] = {(uint32_t)i, sweep, stage, 0u}, r[4];
+        orc_philox4x32_10(ctr, key, r);
+        double zz = 1.0 - 2.0 * orc_u01(r[0]);
+        double phi = 2.0 * M_PI * orc_u01(r[1]);
+        double s = sqrt(fmax(0.0, 1.0 - zz * zz));
+        double mn[3] = {s * cos(phi), s * sin(phi), zz};
+        dE[i] = orc_delta_e(c, c->h, (int)i, mn);
+    }
+}
+
 long orc_sweep(orc_ctx *c, double T, uint32_t stage, uint32_t sweep)
 {
     const long N = c->N;

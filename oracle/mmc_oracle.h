@@ -76,6 +76,8 @@ void     orc_energy(orc_ctx *c, double out[9]);          /* E, 5 terms, <m> */
 void     orc_energy_density(orc_ctx *c, int terms, double *e); /* N, J/m^3 */
 double   orc_delta_e(orc_ctx *c, const double *h, int cell,
                      const double mnew[3]);
+void     orc_trial_delta_e(orc_ctx *c, uint32_t stage, uint32_t sweep,
+                           double *dE);                  /* N, J */
 long     orc_sweep(orc_ctx *c, double T, uint32_t stage, uint32_t sweep);
 int      orc_num_colors(const orc_ctx *c);
 int      orc_cell_color(const orc_ctx *c, int x, int y, int z);

@@ -82,6 +82,8 @@ def lib():
         L.orc_energy_density.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_delta_e.argtypes = [C.c_void_p, dp, C.c_int, dp]
         L.orc_delta_e.restype = C.c_double
+        L.orc_trial_delta_e.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, dp]
+        L.orc_trial_delta_e.restype = None
         L.orc_sweep.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.c_uint32]
         L.orc_sweep.restype = C.c_long
         L.orc_num_colors.argtypes = [C.c_void_p]
@@ -205,6 +207,11 @@ class Oracle:
         h = np.ascontiguousarray(h, dtype=np.float64).reshape(3 * self.N)
         mn = np.ascontiguousarray(mnew, dtype=np.float64)
         return lib().orc_delta_e(self._c, _dp(h), cell, _dp(mn))
+
+    def trial_delta_e(self, stage: int, sweep: int) -> np.ndarray:
+        dE = np.empty(self.N)
+        lib().orc_trial_delta_e(self._c, stage, sweep, _dp(dE))
+        return dE.reshape(self.n[2], self.n[1], self.n[0])
 
     def sweep(self, T: float, stage: int, sweep: int) -> int:
         return lib().orc_sweep(self._c, T, stage, sweep)

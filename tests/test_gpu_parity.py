@@ -281,3 +281,107 @@ def test_khat_planes_stored_compactly(mmc, precision):
     K = np.random.default_rng(7).standard_normal(c.tensor().shape)
     c.set_tensor(K)
     assert c.kernel_bytes("ymul") == spec + 12 * plane
+
+
+@pytest.mark.parametrize("name,geo", CASES, ids=[c[0] for c in CASES])
+def test_trial_delta_e_fp64(orc, mmc, name, geo):
+    """dE of every cell's trial move (the quantity the Metropolis kernels
+    test) matches the oracle's within 1e-12 of its largest value."""
+    kw = film(**geo) if "3d" not in name else dict(FILM3D, **geo)
+    o, c = pair(orc, mmc, **kw)
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 12)
+    o.set_m(m)
+    c.set_m(m)
+    for stage, sweep in ((0, 0), (5, 77)):
+        assert rel_inf(c.trial_delta_e(stage, sweep), o.trial_delta_e(stage, sweep)) <= 1e-12
+
+
+def test_fp32_delta_e_bound(orc, mmc):
+    """FP32-bulk dE against the FP64 oracle on the 256x256 periodic film
+    (20 images): within 2e-6 of the largest |dE| of the sweep."""
+    kw = film(nx=256, ny=256, nz=1, pbc_x=20, pbc_y=20)
+    o, c = pair(orc, mmc, precision=1, shared_tensor=False, **kw)
+    c32 = c
+    c32.build_tensor()
+    o.set_tensor(c32.tensor())
+    m = random_m((256, 256, 1), 21)
+    o.set_m(m)
+    c32.set_m(m)
+    err = rel_inf(c32.trial_delta_e(1, 3), o.trial_delta_e(1, 3))
+    print(f"fp32 dE rel err {err:.3e}")
+    assert err <= 2e-6
+
+
+def _anneal(mmc, kws, precision, sweeps_per_stage, stages):
+    b = mmc.Context([mmc.make_params(**kw) for kw in kws], precision)
+    b.build_tensor()
+    b.init_random_m()
+    b.set_option("graph", 1)
+    for st, T in enumerate(stages):
+        b.set_temperature(T)
+        b.sweeps(st, 0, sweeps_per_stage)
+    return b
+
+
+def _stages(T0=1400.0, T1=40.0, f=0.9):
+    out, T = [], T0
+    while T > T1:
+        out.append(T)
+        T *= f
+    return out
+
+
+def test_anneal_fp64_matches_oracle(orc, mmc):
+    """Whole anneal (1400 K -> x0.9 -> 40 K, 34 stages) on the 32x32 open
+    film: the FP64 library follows the oracle's Markov chain sweep for sweep
+    (same Philox moves, same colour order), so final states agree to
+    rounding."""
+    stages = _stages()
+    assert len(stages) == 34
+    for seed in (1, 2):
+        kw = film(nx=32, ny=32, nz=1, seed=seed)
+        o = orc.Oracle(**kw)
+        o.build_tensor()
+        o.init_random_m(seed)
+        c = mmc.Context(mmc.make_params(**kw), 0)
+        c.set_tensor(o.tensor())
+        c.init_random_m()
+        for st, T in enumerate(stages):
+            c.set_temperature(T)
+            c.sweeps(st, 0, 12)
+            for k in range(12):
+                o.sweep(T, st, k)
+        assert rel_inf(c.m(), o.m()) <= 1e-9
+        assert abs(c.energy()["total"] - o.energy()["total"]) <= 1e-9 * abs(o.energy()["total"])
+
+
+def test_anneal_statistics_16_seeds(orc, mmc):
+    """16 seeds of the full anneal of the 32x32 open film (300 sweeps per
+    stage): FP32-bulk and FP64 final energy density and <m_z> agree within
+    four combined standard errors of the seed mean; the FP32 path's energy of
+    its own final states is within 1e-6 of the FP64 oracle's (the final-energy
+    error of the arithmetic, target 1e-4)."""
+    stages = _stages()
+    kws = [film(nx=32, ny=32, nz=1, seed=s) for s in range(1, 17)]
+    V = 32 * 32 * kws[0]["dx"] * kws[0]["dy"] * kws[0]["dz"]
+    res = {}
+    for prec in (0, 1):
+        b = _anneal(mmc, kws, prec, 300, stages)
+        e = np.array([b.energy(i)["total"] / V for i in range(16)])
+        mz = np.array([b.energy(i)["m_avg"][2] for i in range(16)])
+        res[prec] = (e, mz, [b.m(i) for i in range(16)])
+    for k, name in ((0, "energy density"), (1, "<m_z>")):
+        a, f = res[0][k], res[1][k]
+        se = np.sqrt(a.var(ddof=1) / 16 + f.var(ddof=1) / 16)
+        print(f"{name}: fp64 {a.mean():.6e} +- {a.std(ddof=1) / 4:.2e}, "
+              f"fp32 {f.mean():.6e} +- {f.std(ddof=1) / 4:.2e}")
+        assert abs(a.mean() - f.mean()) <= 4 * se + 1e-12 * abs(a.mean())
+    worst = 0.0
+    for i in (0, 7, 15):
+        o = orc.Oracle(**kws[i])
+        o.build_tensor()
+        o.set_m(res[1][2][i])
+        e64 = o.energy()["total"] / V
+        worst = max(worst, abs(res[1][0][i] - e64) / abs(e64))
+    print(f"fp32 final-energy evaluation error {worst:.3e}")
+    assert worst <= 1e-6

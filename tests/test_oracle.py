@@ -337,6 +337,33 @@ def test_delta_e_equals_energy_difference(orc, geo):
         assert abs(dE - (E1 - E0)) <= 1e-9 * abs(E1 - E0) + 1e-32, (trial, dE, E1 - E0)
 
 
+def test_trial_delta_e_is_the_swept_move(orc):
+    """orc_trial_delta_e gives, per cell, the energy change of the very move a
+    sweep proposes: a sweep at T = 1e300 accepts every move, so its result is
+    the field of trial directions; setting one cell to its trial direction and
+    recomputing the energy reproduces dE."""
+    kw = film(nx=10, ny=7, nz=2, pbc_y=1)
+    n = (10, 7, 2)
+    m = random_m(n, 4)
+    o = orc.Oracle(**kw)
+    o.build_tensor()
+    o.set_m(m)
+    dE = o.trial_delta_e(3, 11).reshape(-1)
+    t = orc.Oracle(**kw)
+    t.build_tensor()
+    t.set_m(m)
+    t.sweep(1e300, 3, 11)
+    trial = t.m().reshape(3, -1)
+    E0 = o.energy()["total"]
+    for cell in (0, 13, 69, 70, 139):
+        m2 = m.reshape(3, -1).copy()
+        m2[:, cell] = trial[:, cell]
+        o.set_m(m2)
+        dEr = o.energy()["total"] - E0
+        assert abs(dE[cell] - dEr) <= 1e-9 * abs(dEr) + 1e-32, (cell, dE[cell], dEr)
+        o.set_m(m)
+
+
 def test_even_periodic_nyquist_slot_is_one_sided(orc):
     """The reference's index map (src/demag_cu.cu:149-151) stores offset +n/2
     in the Nyquist slot of an even periodic axis, image-summed over -p..p.
