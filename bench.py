@@ -74,10 +74,12 @@ def make_ctx(mod, kws, precision, T=1400.0):
 
 
 def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=None,
-            default_stream=False):
-    """Warm up, time `steps` sweeps (hipGraph replay when `graph`), then time
-    `ksteps` more sweeps with every launch bracketed by dispatch-packet HIP
-    events for the per-kernel durations."""
+            default_stream=False, autotune=20):
+    """Pick the sweep schedule with the library's autotuner (`autotune`
+    sweeps per candidate; 0 keeps the default), warm up, time `steps` sweeps
+    (hipGraph replay when `graph`), then time `ksteps` more sweeps with every
+    launch bracketed by dispatch-packet HIP events for the per-kernel
+    durations."""
     if isinstance(kws, dict):
         kws = [kws]
     c, t_tensor = make_ctx(mod, kws, precision, T)
@@ -88,6 +90,11 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
     if graph:
         c.set_option("graph", 1)
         c.set_option("graph_sweeps", 20)
+    schedule = None
+    if autotune:
+        rep = c.autotune(autotune)
+        log(rep.rstrip())
+        schedule = rep.strip().splitlines()[-1].replace("chosen: ", "")
     c.sweeps(0, 0, warmup)
     c.sync()
     t0 = time.perf_counter()
@@ -102,7 +109,7 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
         E = c.energy()
         return dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
                     eager_timed_attempts_per_s=None, tensor_build_s=t_tensor, kernels=kern,
-                    energy_J=E["total"], ctx=c, cells=N)
+                    energy_J=E["total"], ctx=c, cells=N, schedule=schedule)
     c.set_option("timing", 1)
     for k in mod.KINDS:
         c.kernel_stats(k, reset=True)
@@ -119,7 +126,8 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
     E = c.energy()
     return dict(attempts_per_s=N * steps / dt, ms_per_sweep=1e3 * dt / steps,
                 eager_timed_attempts_per_s=N * ksteps / dt_eager,
-                tensor_build_s=t_tensor, kernels=kern, energy_J=E["total"], ctx=c, cells=N)
+                tensor_build_s=t_tensor, kernels=kern, energy_J=E["total"], ctx=c, cells=N,
+                schedule=schedule)
 
 
 def batch_params(count=64, master_seed=2024):
@@ -137,6 +145,23 @@ def batch_params(count=64, master_seed=2024):
                         bext_y=0.0, bext_z=float(rng.uniform(0.0, 0.6)), cal_demag=1,
                         seed=1000 + i))
     return out
+
+
+def copy_calibration(nbytes, times):
+    """Device-to-device copies of a known size through the HIP runtime the
+    library links (hipMemcpy runs the runtime's copy kernel): the reference
+    point for scaling hardware byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(times):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def roofline(kern):
@@ -188,6 +213,9 @@ def main():
     ap.add_argument("--default-stream", action="store_true",
                     help="launch on the device's default stream (hardware-counter runs; "
                          "implies --no-graph)")
+    ap.add_argument("--autotune", type=int, default=20,
+                    help="sweeps per candidate schedule for the library's autotuner "
+                         "(0: keep the default schedule)")
     ap.add_argument("--ksteps", type=int, default=None,
                     help="event-timed eager sweeps after the timed region (default "
                          "min(steps, 400); 0 skips them, as hardware-counter runs need)")
@@ -201,24 +229,19 @@ def main():
     mod = load_pkg()
     mod.lib()
     if a.copy_calibration:
-        import torch
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        copy_calibration(268435456, 3)
         log("[bench] calibration: 3 device copies of 268435456 bytes")
-        del src, dst
 
     kw = film256() if a.workload == "film256" else film1024x2()
     log(f"[bench] {a.workload} FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
     r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps,
-                  default_stream=a.default_stream)
+                  default_stream=a.default_stream, autotune=a.autotune)
     dom, roof = roofline(r32["kernels"])
     extra = {}
     if not a.no_extra:
         log("[bench] same film, plain FP64")
-        r64 = run_gpu(mod, kw, mod.FP64, max(200, a.steps // 2), max(20, a.warmup // 4))
+        r64 = run_gpu(mod, kw, mod.FP64, max(200, a.steps // 2), max(20, a.warmup // 4),
+                      autotune=a.autotune)
         extra["fp64_attempts_per_s"] = r64["attempts_per_s"]
         extra["fp64_ms_per_sweep"] = r64["ms_per_sweep"]
         extra["fp64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r64["kernels"].items()}
@@ -226,29 +249,31 @@ def main():
         r64["ctx"].close()
         log("[bench] 64x64x1 open film, plain FP64 single instance")
         kw0 = film256(nx=64, ny=64, pbc_x=0, pbc_y=0)
-        r0 = run_gpu(mod, kw0, mod.FP64, max(400, a.steps), a.warmup)
+        r0 = run_gpu(mod, kw0, mod.FP64, max(400, a.steps), a.warmup, autotune=a.autotune)
         extra["film64_fp64_attempts_per_s"] = r0["attempts_per_s"]
         extra["film64_fp64_ms_per_sweep"] = r0["ms_per_sweep"]
         r0["ctx"].close()
         log("[bench] 64 uneven films batched, FP32-bulk")
         bp = batch_params()
-        rb = run_gpu(mod, bp, mod.FP32, 200, 20)
+        rb = run_gpu(mod, bp, mod.FP32, 200, 20, autotune=a.autotune)
         db, roofb = roofline(rb["kernels"])
         extra["batch64_fp32_attempts_per_s"] = rb["attempts_per_s"]
         extra["batch64_ms_per_sweep"] = rb["ms_per_sweep"]
         extra["batch64_cells"] = rb["cells"]
         extra["batch64_roofline"] = roofb
+        extra["batch64_schedule"] = rb["schedule"]
         extra["batch64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in rb["kernels"].items()}
         extra["batched_fp32_over_single_fp64"] = rb["attempts_per_s"] / r0["attempts_per_s"]
         rb["ctx"].close()
         kw4 = film1024x2()
         log("[bench] 1024x1024x2 PBC10 FP32-bulk")
-        r4 = run_gpu(mod, kw4, mod.FP32, 200, 20)
+        r4 = run_gpu(mod, kw4, mod.FP32, 200, 20, autotune=a.autotune)
         d4, roof4 = roofline(r4["kernels"])
         extra["film1024x2_attempts_per_s"] = r4["attempts_per_s"]
         extra["film1024x2_ms_per_sweep"] = r4["ms_per_sweep"]
         extra["film1024x2_tensor_build_s"] = r4["tensor_build_s"]
         extra["film1024x2_roofline"] = roof4
+        extra["film1024x2_schedule"] = r4["schedule"]
         extra["film1024x2_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r4["kernels"].items()}
         r4["ctx"].close()
     log("[bench] CPU baseline (oracle)")
@@ -273,6 +298,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in r32["kernels"].items()},
+        "schedule": r32["schedule"],
         "eager_timed_attempts_per_s": r32["eager_timed_attempts_per_s"],
         "tensor_build_s": r32["tensor_build_s"],
     }

@@ -20,6 +20,7 @@
 #ifndef MMC_H
 #define MMC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -155,6 +156,17 @@ int mmc_set_temperature(mmc_ctx *ctx, int inst, double T);
  * loop body of src/main.cu:868-884 + :1101-1120 (R2C x3, calHexchangeKernel,
  * mcKernel2, M/M_out swap). */
 int mmc_sweeps(mmc_ctx *ctx, uint32_t stage, uint32_t sweep0, uint32_t count);
+
+/* Time every admissible sweep schedule — fused x-pass + Metropolis kernels
+ * or separate launches or the tiled two-colour Metropolis kernel, each with
+ * launch-sized, minimal and widest FFT tiles — over sweeps_per_trial sweeps
+ * (twice: once untimed) and keep the fastest.  Every schedule gives the
+ * bit-identical Markov chain; the magnetisation is restored afterwards.  A
+ * text table of the timings (and the choice) is written to report when it
+ * is non-NULL.  No reference counterpart: the reference has one fixed
+ * launch sequence (src/main.cu:868-884, :1101-1120). */
+int mmc_autotune(mmc_ctx *ctx, int sweeps_per_trial, char *report,
+                 size_t report_len);
 
 /* Single-instance convenience: set T, run one sweep. */
 int mmc_sweep(mmc_ctx *ctx, double T, uint32_t stage, uint32_t sweep);

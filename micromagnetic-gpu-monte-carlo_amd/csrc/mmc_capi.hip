@@ -9,6 +9,7 @@
 #include "fft_fixed.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -119,6 +120,9 @@ struct mmc_ctx {
     bool metro_tiled_ok = false;      // every instance admits the tiled kernel
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
     bool fuse_ok = false;             // every instance admits them
+    // per instance (tyx, ncol, tyf): widest tiles that fit, and the tiles
+    // chosen for the launch sizes (the autotuner's starting points)
+    std::vector<std::array<int, 3>> tile_full, tile_default;
     int max_colors = 2;
     bool any_demag = false;
     bool tensor_ready = false;
@@ -716,7 +720,12 @@ int create_common(const mmc_params *p, int count, int precision, mmc_ctx **out)
         *out = c;
         return fail(c, MMC_EHIP, "hipStreamCreate failed");
     }
+    for (auto &in : c->inst) {
+        in.tyf = fused_full(c, in);
+        c->tile_full.push_back({in.tyx, in.ncol, in.tyf});
+    }
     adapt_tiles(c);
+    for (auto &in : c->inst) c->tile_default.push_back({in.tyx, in.ncol, in.tyf});
     build_lists(c);
     rc = alloc_all(c);
     *out = c;
@@ -1098,6 +1107,107 @@ int energy_impl(mmc_ctx *c, std::vector<double> &sums)
     return MMC_OK;
 }
 
+// ---- autotuner --------------------------------------------------------------
+// A sweep schedule: fused x + Metropolis passes or separate launches (or the
+// tiled two-colour Metropolis kernel), and the tile family.
+struct Schedule {
+    const char *name;
+    bool fuse, tiled;
+    int tiles;          // 0 chosen for the launch sizes, 1 minimal, 2 widest
+};
+
+int apply_schedule(mmc_ctx *c, const Schedule &s)
+{
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        Inst &in = c->inst[i];
+        const auto &d = s.tiles == 2 ? c->tile_full[i] : c->tile_default[i];
+        in.tyx = d[0];
+        in.ncol = d[1];
+        in.tyf = c->fuse_ok ? d[2] : 0;
+        if (s.tiles == 1) {
+            in.tyx = 2;
+            in.ncol = 1;
+            in.tyf = in.tyf ? 1 : 0;
+        }
+    }
+    c->fuse = s.fuse;
+    c->metro_tiled = s.tiled;
+    return retile(c, 0, 0);
+}
+
+// Device time of `sweeps` sweeps under the current schedule (one untimed
+// run first, which also captures the graph in graph mode).
+template <typename T>
+int time_sweeps(mmc_ctx *c, int sweeps, double *us)
+{
+    const uint32_t stage = 0xFFFFFFFFu;           // state is restored afterwards
+    int rc = run_sweeps<T>(c, stage, 0, (uint32_t)sweeps);
+    if (rc) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHK(c, hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, c->stream);
+    if (e == hipSuccess) {
+        rc = run_sweeps<T>(c, stage, (uint32_t)sweeps, (uint32_t)sweeps);
+        if (!rc) e = hipEventRecord(b, c->stream);
+    }
+    if (e == hipSuccess && !rc) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(c, MMC_EHIP, "autotune timing: %s", hipGetErrorString(e));
+    *us = 1e3 * ms / sweeps;
+    return MMC_OK;
+}
+
+template <typename T>
+int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
+{
+    static const Schedule all[] = {
+        {"fused, launch-sized tiles", true, false, 0},
+        {"fused, minimal tiles", true, false, 1},
+        {"fused, widest tiles", true, false, 2},
+        {"separate, launch-sized tiles", false, false, 0},
+        {"separate, minimal tiles", false, false, 1},
+        {"separate, widest tiles", false, false, 2},
+        {"tiled two-colour Metropolis", false, true, 0},
+    };
+    void *save = nullptr;
+    const size_t mbytes = std::max<size_t>(1, c->m_elems) * c->tsize;
+    HIPCHK(c, hipMalloc(&save, mbytes));
+    hipError_t e = hipMemcpyAsync(save, c->d_m, mbytes, hipMemcpyDeviceToDevice, c->stream);
+    const bool timing = c->timing;
+    c->timing = false;
+    int rc = e == hipSuccess ? MMC_OK : fail(c, MMC_EHIP, "autotune: %s", hipGetErrorString(e));
+    int best = -1;
+    double best_us = 0;
+    char line[160];
+    for (int k = 0; k < (int)(sizeof(all) / sizeof(all[0])) && !rc; ++k) {
+        const Schedule &s = all[k];
+        if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok)) continue;
+        rc = apply_schedule(c, s);
+        double us = 0;
+        if (!rc) rc = time_sweeps<T>(c, sweeps, &us);
+        if (rc) break;
+        snprintf(line, sizeof(line), "%-32s %10.3f us/sweep\n", s.name, us);
+        report += line;
+        if (best < 0 || us < best_us) { best = k; best_us = us; }
+    }
+    if (!rc && best >= 0) {
+        rc = apply_schedule(c, all[best]);
+        snprintf(line, sizeof(line), "chosen: %s\n", all[best].name);
+        report += line;
+    }
+    e = hipMemcpyAsync(c->d_m, save, mbytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(save);
+    c->timing = timing;
+    if (!rc && e != hipSuccess) rc = fail(c, MMC_EHIP, "autotune restore: %s", hipGetErrorString(e));
+    return rc;
+}
+
 }  // namespace
 
 // =============================================================================
@@ -1384,6 +1494,24 @@ int mmc_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
     if (!c) return MMC_EINVAL;
     return c->tsize == 4 ? run_sweeps<float>(c, stage, sweep0, count)
                          : run_sweeps<double>(c, stage, sweep0, count);
+}
+
+int mmc_autotune(mmc_ctx *c, int sweeps_per_trial, char *report, size_t report_len)
+{
+    if (!c) return MMC_EINVAL;
+    if (sweeps_per_trial < 1 || sweeps_per_trial > 100000)
+        return fail(c, MMC_EINVAL, "sweeps_per_trial must be in [1,100000]");
+    if (c->any_demag && !c->tensor_ready)
+        return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
+    std::string rep;
+    const int rc = c->tsize == 4 ? autotune_impl<float>(c, sweeps_per_trial, rep)
+                                 : autotune_impl<double>(c, sweeps_per_trial, rep);
+    if (report && report_len) {
+        const size_t n = std::min(report_len - 1, rep.size());
+        std::memcpy(report, rep.data(), n);
+        report[n] = 0;
+    }
+    return rc;
 }
 
 int mmc_sweep(mmc_ctx *c, double T, uint32_t stage, uint32_t sweep)

@@ -401,19 +401,13 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
     }
 }
 
-// Direct DFT along z of one y column (element z at p[z*stride]): nin inputs
-// (zero above), nout outputs, length P <= kZDirect, forward or inverse
-// (conjugate twiddles).  In place safe: all inputs are read first.
+// Direct DFT of length P <= kZDirect held in registers: nin inputs (zero
+// above), nout outputs, forward or inverse (conjugate twiddles).
 template <typename T, bool INV>
-__device__ __forceinline__ void zdft_column(const cpx<T> *src, cpx<T> *dst, int stride,
-                                            int nin, int nout, int P,
-                                            const cpx<T> *__restrict__ tw)
+__device__ __forceinline__ void zdft_regs(const cpx<T> (&v)[kZDirect], cpx<T> (&out)[kZDirect],
+                                          int nin, int nout, int P,
+                                          const cpx<T> *__restrict__ tw)
 {
-    cpx<T> v[kZDirect];
-    #pragma unroll
-    for (int z = 0; z < kZDirect; ++z)
-        v[z] = z < nin ? src[(long)z * stride] : cpx<T>{T(0), T(0)};
-    cpx<T> out[kZDirect];
     #pragma unroll
     for (int k = 0; k < kZDirect; ++k) {
         cpx<T> acc = v[0];
@@ -427,6 +421,21 @@ __device__ __forceinline__ void zdft_column(const cpx<T> *src, cpx<T> *dst, int 
         }
         out[k] = acc;
     }
+}
+
+// Direct DFT along z of one y column (element z at p[z*stride]): nin inputs
+// (zero above), nout outputs, length P <= kZDirect, forward or inverse
+// (conjugate twiddles).  In place safe: all inputs are read first.
+template <typename T, bool INV>
+__device__ __forceinline__ void zdft_column(const cpx<T> *src, cpx<T> *dst, int stride,
+                                            int nin, int nout, int P,
+                                            const cpx<T> *__restrict__ tw)
+{
+    cpx<T> v[kZDirect], out[kZDirect];
+    #pragma unroll
+    for (int z = 0; z < kZDirect; ++z)
+        v[z] = z < nin ? src[(long)z * stride] : cpx<T>{T(0), T(0)};
+    zdft_regs<T, INV>(v, out, nin, nout, P, tw);
     #pragma unroll
     for (int k = 0; k < kZDirect; ++k)
         if (k < nout) dst[(long)k * stride] = out[k];

@@ -300,9 +300,9 @@ def test_fp32_delta_e_bound(orc, mmc):
     """FP32-bulk dE against the FP64 oracle on the 256x256 periodic film
     (20 images): within 2e-6 of the largest |dE| of the sweep."""
     kw = film(nx=256, ny=256, nz=1, pbc_x=20, pbc_y=20)
-    o, c = pair(orc, mmc, precision=1, shared_tensor=False, **kw)
-    c32 = c
+    c32 = mmc.Context(mmc.make_params(**kw), 1)
     c32.build_tensor()
+    o = orc.Oracle(**kw)
     o.set_tensor(c32.tensor())
     m = random_m((256, 256, 1), 21)
     o.set_m(m)
@@ -385,3 +385,24 @@ def test_anneal_statistics_16_seeds(orc, mmc):
         worst = max(worst, abs(res[1][0][i] - e64) / abs(e64))
     print(f"fp32 final-energy evaluation error {worst:.3e}")
     assert worst <= 1e-6
+
+
+@pytest.mark.parametrize("geo", [dict(nx=128, ny=96, nz=1, pbc_x=2),
+                                 dict(nx=15, ny=12, nz=1, pbc_x=1),
+                                 dict(nx=64, ny=20, nz=3)],
+                         ids=["fused_pbc", "three_colours", "3d"])
+def test_autotune_keeps_the_chain(mmc, geo):
+    """The autotuner times the admissible schedules, keeps the fastest, and
+    restores the state: the chain afterwards is the untuned one."""
+    kw = film(**geo)
+    ref = _run(mmc, kw, 1, 6)[-1]
+    c = mmc.Context(mmc.make_params(**kw), 1)
+    c.build_tensor()
+    c.init_random_m()
+    c.set_temperature(700.0)
+    c.set_option("graph", 1)
+    rep = c.autotune(4)
+    print(rep)
+    assert "chosen:" in rep
+    c.sweeps(1, 0, 6)
+    assert np.array_equal(c.m(), ref)
