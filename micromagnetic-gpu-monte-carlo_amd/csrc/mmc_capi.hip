@@ -116,6 +116,8 @@ struct mmc_ctx {
     List xl, zl, yl, cl, ml, mtl, tl, txl, tzl, tyl;  // FFT passes, cells, colour slots,
                                                       // Metropolis tiles, tensor passes
     List fl;                          // fused x pass + Metropolis tiles
+    List zls;                         // z passes of fused sweeps (instances
+                                      // whose fused tiles do not fold z in)
     bool metro_tiled = false;         // both colours per launch (k_metro_tile)
     bool metro_tiled_ok = false;      // every instance admits the tiled kernel
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
@@ -359,7 +361,8 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
     // x pass (spectrum) + one checkerboard colour: m read (cell and
     // neighbours), half of m written, half of h written (xinv_metro) or
     // read (metro_xfwd)
-    case K_XINVM: case K_MXFWD: return spec_x + 6 * N * b;
+    case K_XINVM: case K_MXFWD:
+        return in.cal_demag ? (in.zf ? spec_z : spec_x) + 6 * N * b : 0.0;
     case K_TENSOR: return 6.0 * in.NP * 8;
     }
     return 0.0;
@@ -380,17 +383,56 @@ bool xfixed(const mmc_ctx *c, const Inst &in)
            (size_t)(xrows_for(c->tsize) + 1) * in.Px * 2 * c->tsize <= c->lds_max;
 }
 
-// Row pairs of a full fused x + Metropolis tile of instance `in` (0 when the
-// instance cannot take the fused passes: no demag, three colours, an x
-// length without a fixed plan, or LDS too small for one row pair).
-int fused_full(const mmc_ctx *c, const Inst &in)
+// Complex rows per component of a full fused x + Metropolis tile of
+// instance `in` (0 when the instance cannot take the fused passes: no demag,
+// three colours, an x length without a fixed plan, or LDS too small for one
+// row pair).  Sets in.zf when a tile can span every layer and fold the z
+// transform in (Pz <= kZDirect and the layers divide the tile).
+int fused_full(const mmc_ctx *c, Inst &in)
 {
+    in.zf = 0;
     if (!in.cal_demag || in.ncolors != 2 || !xfixed_size(in.Px)) return 0;
     const size_t cs = 2 * c->tsize;
     const int r = fused_rows(c->tsize == 4 ? fused_rows_base<float>() : fused_rows_base<double>(),
                              in.Px);
-    if ((3 * (size_t)r + 1) * in.Px * cs <= c->lds_max) return r;
+    const bool fits = (3 * (size_t)r + 1) * in.Px * cs <= c->lds_max;
+    if (fits && in.Pz > 1 && in.Pz <= kZDirect && r % in.nz == 0) {
+        in.zf = 1;
+        return r;
+    }
+    if (fits) return r;
     return (4 * (size_t)in.Px * cs <= c->lds_max) ? 1 : 0;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin by index, and each
+// XCD has its own L2.  x-pass items that touch the same 128-byte lines of a
+// y-contiguous spectrum (consecutive row tiles: `share` items per line) are
+// re-indexed so that a group of `share` items gets indices ≡ g (mod 8), one
+// after another: they run on one XCD at about the same time, and a line
+// comes from HBM once instead of once per item.
+constexpr int kXcds = 8;
+
+void xcd_group(std::vector<Work> &items, int share)
+{
+    if (share <= 1) return;
+    const long n = (long)items.size();
+    const long block = (long)kXcds * share;
+    const long full = n / block * block;
+    std::vector<Work> out(items);
+    for (long j = 0; j < full; ++j) {
+        const long g = j / share, l = j % share;
+        const long sb = g / kXcds, gx = g % kXcds;
+        out[sb * block + l * kXcds + gx] = items[j];
+    }
+    items.swap(out);
+}
+
+// Items of an x-pass tile of `rows` real rows that share one 128-byte line
+// of the spectrum (complex element size cs).
+int line_share(int rows, size_t cs)
+{
+    const int chunk = (int)(rows * cs);
+    return chunk >= 128 ? 1 : 128 / chunk;
 }
 
 void build_lists(mmc_ctx *c)
@@ -403,7 +445,7 @@ void build_lists(mmc_ctx *c)
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
     std::map<std::pair<int, int>, Segment> fclass;   // (x length, row pairs) -> items
     std::map<std::pair<int, int>, std::vector<Work>> fitems;
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl}) reset(*l);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls}) reset(*l);
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
         const bool zdirect = in.Pz <= kZDirect;
@@ -426,7 +468,10 @@ void build_lists(mmc_ctx *c)
             if (in.Pz > 1) {
                 for (int comp = 0; comp < 3; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
-                        for (int y0 = 0; y0 < in.ny; y0 += zstep) c->zl.host.push_back({i, comp, kx, y0});
+                        for (int y0 = 0; y0 < in.ny; y0 += zstep) {
+                            c->zl.host.push_back({i, comp, kx, y0});
+                            if (!in.zf) c->zls.host.push_back({i, comp, kx, y0});
+                        }
                 if (!zdirect) c->zl.lds = std::max(c->zl.lds, (2 * (size_t)in.tyz + 1) * in.Pz * cs);
             }
             const int fn = fixed_fft_size(in.Py) ? in.Py : 0;
@@ -470,18 +515,25 @@ void build_lists(mmc_ctx *c)
         }
         if (in.tyf) {
             const std::pair<int, int> fk(in.Px, in.tyf);
-            for (int z = 0; z < in.nz; ++z)
-                for (int y0 = 0; y0 < in.ny; y0 += 2 * in.tyf) fitems[fk].push_back({i, 0, z, y0});
+            if (in.zf) {
+                const int rows = 2 * (in.tyf / in.nz);
+                for (int y0 = 0; y0 < in.ny; y0 += rows) fitems[fk].push_back({i, 0, 0, y0});
+            } else {
+                for (int z = 0; z < in.nz; ++z)
+                    for (int y0 = 0; y0 < in.ny; y0 += 2 * in.tyf) fitems[fk].push_back({i, 0, z, y0});
+            }
             Segment &fs = fclass[fk];
             fs.fixed_n = in.Px;
             fs.rows = in.tyf;
             fs.lds = std::max(fs.lds, (3 * (size_t)in.tyf + 1) * in.Px * cs);
             fs.bytes += inst_bytes(c, in, K_XINVM);
         }
-        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl})
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls})
             l->first.push_back(l->n());
     }
+    c->zls.lds = c->zl.lds;
     for (auto &kv : fitems) {
+        xcd_group(kv.second, line_share(2 * kv.first.second, cs));
         Segment sg = fclass[kv.first];
         sg.begin = c->fl.n();
         sg.count = (int)kv.second.size();
@@ -490,6 +542,7 @@ void build_lists(mmc_ctx *c)
     }
     // x and y passes: one contiguous segment (and launch) per kernel variant
     for (auto &kv : xitems) {
+        xcd_group(kv.second, line_share(2 * std::max(1, kv.first.second), cs));
         Segment sg = xclass[kv.first];
         sg.begin = c->xl.n();
         sg.count = (int)kv.second.size();
@@ -602,7 +655,7 @@ void adapt_tiles(mmc_ctx *c)
         c->inst, target, [](const Inst &in) { return in.tyf ? in.Px : -1; },
         [](const Inst &in) { return (long)in.nz * ((in.ny + 2 * in.tyf - 1) / (2 * in.tyf)); },
         [](Inst &in) {
-            if (in.tyf <= 1) return false;
+            if (in.tyf <= 1 || in.zf) return false;   // z-folding tiles keep all layers
             in.tyf = 1;
             return true;
         });
@@ -673,7 +726,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
     HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -748,7 +801,7 @@ int retile(mmc_ctx *c, int tyx, int ncol)
         }
     }
     build_lists(c);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -784,21 +837,46 @@ int pass_xfwd(mmc_ctx *c)
     return MMC_OK;
 }
 
-// A -> (z) -> y, Khat product, y -> (z) -> A
+// z pass bytes over the instances list `l` serves (all with Pz > 1, or in
+// fused sweeps those whose tiles do not fold z in)
+double zbytes(const mmc_ctx *c, bool sweep)
+{
+    double t = 0;
+    for (auto &in : c->inst)
+        if (!(sweep && in.zf)) t += inst_bytes(c, in, K_ZFWD);
+    return t;
+}
+
 template <typename T>
-int pass_zyz(mmc_ctx *c)
+int pass_zfwd(mmc_ctx *c, bool sweep)
+{
+    const Bufs<T> b(c);
+    const List &l = sweep ? c->zls : c->zl;
+    if (l.n())
+        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(c->stream, c->dInst, l.dev, l.n(), 0, b.A, b.B, b.tw, l.lds); },
+                        zbytes(c, sweep)));
+    return MMC_OK;
+}
+
+// A -> (z) -> y, Khat product, y -> (z) -> A.  In fused sweeps (`sweep`)
+// instances whose fused tiles fold z in skip the z passes: their tiles
+// read and write B directly.
+template <typename T>
+int pass_zyz(mmc_ctx *c, bool sweep = false)
 {
     const Bufs<T> b(c);
     hipStream_t st = c->stream;
-    if (c->zl.n())
-        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(st, c->dInst, c->zl.dev, c->zl.n(), 0, b.A, b.B, b.tw, c->zl.lds); }));
+    const List &l = sweep ? c->zls : c->zl;
+    int rc = pass_zfwd<T>(c, sweep);
+    if (rc) return rc;
     for (const Segment &sg : c->yl.seg)
         HIPCHK(c, timed(c, K_YMUL, [&] {
                    return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
                                          b.B, b.Kh, b.tw, sg.lds);
                }, sg.bytes));
-    if (c->zl.n())
-        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, c->zl.dev, c->zl.n(), b.B, b.A, b.tw, c->zl.lds); }));
+    if (l.n())
+        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, l.dev, l.n(), b.B, b.A, b.tw, l.lds); },
+                        zbytes(c, sweep)));
     return MMC_OK;
 }
 
@@ -823,16 +901,16 @@ int demag_chain(mmc_ctx *c)
 
 bool fused_sweeps(const mmc_ctx *c) { return c->fuse && !c->metro_tiled; }
 
-// One sweep.  Fused mode expects A to hold the x spectrum of the current
-// state (run_sweeps launches pass_xfwd first) and leaves it holding the
-// spectrum of the new state: z/y/z passes, then inverse x + colour 0, then
-// colour 1 + forward x.
+// One sweep.  Fused mode expects A (B for z-folding tiles) to hold the
+// spectrum of the current state (run_sweeps prepares it) and leaves it
+// holding the spectrum of the new state: z/y/z passes, then inverse x +
+// colour 0, then colour 1 + forward x.
 template <typename T>
 int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
 {
     hipStream_t st = c->stream;
     if (fused_sweeps(c)) {
-        int rc = pass_zyz<T>(c);
+        int rc = pass_zyz<T>(c, true);
         if (rc) return rc;
         const Bufs<T> b(c);
         for (const Segment &sg : c->fl.seg)
@@ -878,7 +956,9 @@ int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
     if (c->any_demag && !c->tensor_ready)
         return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
     if (count && fused_sweeps(c)) {
+        // x spectrum of the current state (xz spectrum for z-folding tiles)
         int rc = pass_xfwd<T>(c);
+        if (!rc && c->zls.n() != c->zl.n()) rc = pass_zfwd<T>(c, false);
         if (rc) return rc;
     }
     if (!c->use_graph || c->timing) {
@@ -1127,7 +1207,7 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
         if (s.tiles == 1) {
             in.tyx = 2;
             in.ncol = 1;
-            in.tyf = in.tyf ? 1 : 0;
+            in.tyf = in.tyf && !in.zf ? 1 : in.tyf;
         }
     }
     c->fuse = s.fuse;
@@ -1236,7 +1316,7 @@ void mmc_destroy(mmc_ctx *c)
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
         if (p) (void)hipFree(p);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl})
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls})
         if (l->dev) (void)hipFree(l->dev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1625,7 +1705,7 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         // row pairs per fused tile: 1, or (value > 1) the full tile
         if (value < 1) return fail(c, MMC_EINVAL, "fuse_rows must be >= 1");
         for (auto &in : c->inst)
-            if (in.tyf) in.tyf = value > 1 ? fused_full(c, in) : 1;
+            if (in.tyf) in.tyf = (value > 1 || in.zf) ? fused_full(c, in) : 1;
         return retile(c, 0, 0);
     }
     if (k == "fuse") {

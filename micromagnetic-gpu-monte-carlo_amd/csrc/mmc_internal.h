@@ -47,7 +47,9 @@ struct Inst {
                           // pass ping-pongs two LDS buffers of tile size
     int tyx64, tyz64;     // tile shapes of the FP64 tensor transform
     int mtx, mty;         // Metropolis x-y tile (both colours in one pass)
-    int tyf;              // row pairs per fused x-pass + Metropolis tile (0: none)
+    int tyf;              // complex rows per component of a fused x-pass +
+                          // Metropolis tile (0: none)
+    int zf;               // fused tiles span all layers and transform z too
     int kmode[6];         // Khat storage of each component: 0 zero, 1 real
                           // (imaginary part below rounding), 2 complex
     long N, NP;

@@ -1054,13 +1054,19 @@ __device__ __forceinline__ V3<T> metro_update(const Inst &in, T *m, int x, int y
     return mi;
 }
 
-// Inverse x pass of all three components of 2R real rows (item (inst, 0, z,
-// y0): rows y0.. of layer z) fused with the first checkerboard colour of
-// those rows.  The h rows stay in LDS for colour 0 (cells with x + y + z
-// even); the h of colour-1 cells is stored compactly for the second pass:
-// hc[comp][(z*ny + y)*hw + j], x = 2j + ((y + z + 1) & 1), hw = ceil(nx/2),
-// in the instance's h region.  Same arithmetic as k_xinv_n + k_metro.
-template <typename T, int N, int R>
+// Inverse x pass of all three components of a tile of real rows fused with
+// the first checkerboard colour of those rows.  Item (inst, 0, z0, y0): RZ
+// complex rows per component, each two real rows packed.  An instance with
+// zf = 0 covers layer z0, rows y0.. (R = RZ row pairs) and reads the
+// x-spectrum A; with zf = 1 (short z, Pz <= kZDirect) it covers all nz
+// layers (R = RZ / nz row pairs per layer) and reads the xz-spectrum B,
+// inverting z in registers first.  LDS row (comp, layer zl, pair q) =
+// comp*RZ + zl*R + q.  The h rows stay in LDS for colour 0 (cells with
+// x + y + z even); the h of colour-1 cells is stored compactly for the
+// second pass: hc[comp][(z*ny + y)*hw + j], x = 2j + ((y + z + 1) & 1),
+// hw = ceil(nx/2), in the instance's h region.  Same arithmetic as
+// k_zinv + k_xinv_n + k_metro.
+template <typename T, int N, int RZ>
 __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict__ I,
                                                          const Work *__restrict__ W,
                                                          const cpx<T> *__restrict__ specb,
@@ -1070,47 +1076,81 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
                                                          uint32_t sweep_off,
                                                          const cpx<T> *__restrict__ twb)
 {
-    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);    // row comp*R + pair
-    cpx<T> *tw = s + 3 * R * N;
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + 3 * RZ * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
-    const int z = w.b, y0 = w.c;
+    const int z0 = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const int nzl = in.zf ? nz : 1;
+    const int R = RZ / nzl;
     const int nyt = min(2 * R, ny - y0);
     const int npair = (nyt + 1) >> 1;
-    const long cstride = (long)H * nz * ny;
-    const cpx<T> *src = specb + in.a_off;
     for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
     if (npair < R) {
         const int zr = (R - npair) * N;
-        for (int e = threadIdx.x; e < 3 * zr; e += kThreads) {
-            const int c = e / zr, r = e - c * zr;
-            s[(c * R + npair) * N + r] = {T(0), T(0)};
+        for (int e = threadIdx.x; e < 3 * nzl * zr; e += kThreads) {
+            const int cl = e / zr, r = e - cl * zr;
+            const int c = cl / nzl, zl = cl - c * nzl;
+            s[(c * RZ + zl * R + npair) * N + r] = {T(0), T(0)};
         }
     }
     const int per = H * npair;
-    for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
-        const int c = e / per, r = e - c * per;
-        const int k = r / npair, q = r - k * npair;
-        const cpx<T> *p = src + c * cstride + ((long)k * nz + z) * ny + y0 + 2 * q;
-        const cpx<T> a = p[0];
-        T bx = T(0), by = T(0);
-        if (2 * q + 1 < nyt) {
-            const cpx<T> b = p[1];
-            bx = b.x;
-            by = b.y;
+    if (!in.zf) {
+        const long cstride = (long)H * nz * ny;
+        const cpx<T> *src = specb + in.a_off;
+        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+            const int c = e / per, r = e - c * per;
+            const int k = r / npair, q = r - k * npair;
+            const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
+            const cpx<T> a = p[0];
+            T bx = T(0), by = T(0);
+            if (2 * q + 1 < nyt) {
+                const cpx<T> b = p[1];
+                bx = b.x;
+                by = b.y;
+            }
+            cpx<T> *row = s + (c * RZ + q) * N;
+            row[k] = {a.x - by, -(a.y + bx)};
+            if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
         }
-        cpx<T> *row = s + (c * R + q) * N;
-        row[k] = {a.x - by, -(a.y + bx)};
-        if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
+    } else {
+        const int Pz = in.Pz;
+        const long cstride = (long)H * Pz * ny;
+        const cpx<T> *src = specb + in.b_off;
+        const cpx<T> *twz = twb + in.twz;
+        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+            const int c = e / per, r = e - c * per;
+            const int k = r / npair, q = r - k * npair;
+            const cpx<T> *p = src + c * cstride + (long)k * Pz * ny + y0 + 2 * q;
+            const bool hasb = 2 * q + 1 < nyt;
+            cpx<T> va[kZDirect], vb[kZDirect], oa[kZDirect], ob[kZDirect];
+            #pragma unroll
+            for (int kz = 0; kz < kZDirect; ++kz) {
+                va[kz] = kz < Pz ? p[(long)kz * ny] : cpx<T>{T(0), T(0)};
+                vb[kz] = (kz < Pz && hasb) ? p[(long)kz * ny + 1] : cpx<T>{T(0), T(0)};
+            }
+            zdft_regs<T, true>(va, oa, Pz, nz, Pz, twz);
+            zdft_regs<T, true>(vb, ob, Pz, nz, Pz, twz);
+            #pragma unroll
+            for (int zl = 0; zl < kZDirect; ++zl) {
+                if (zl < nz) {
+                    const cpx<T> a = oa[zl];
+                    const T bx = hasb ? ob[zl].x : T(0), by = hasb ? ob[zl].y : T(0);
+                    cpx<T> *row = s + (c * RZ + zl * R + q) * N;
+                    row[k] = {a.x - by, -(a.y + bx)};
+                    if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
+                }
+            }
+        }
     }
     __syncthreads();
-    fft_fixed<T, N, 3 * R>(s, tw);
-    auto hval = [&](int yy, int x) -> V3<T> {
-        const int o = (yy >> 1) * N + x;
-        if (yy & 1) return {-s[o].y, -s[R * N + o].y, -s[2 * R * N + o].y};
-        return {s[o].x, s[R * N + o].x, s[2 * R * N + o].x};
+    fft_fixed<T, N, 3 * RZ>(s, tw);
+    auto hval = [&](int zl, int yy, int x) -> V3<T> {
+        const int o = (zl * R + (yy >> 1)) * N + x;
+        if (yy & 1) return {-s[o].y, -s[RZ * N + o].y, -s[2 * RZ * N + o].y};
+        return {s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
     };
     T *m = mb + in.m_off;
     T *hc = hb + in.h_off;
@@ -1119,13 +1159,16 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
     const long cst = (long)hw * ny * nz;
     const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
     const T bt = T(beta[w.inst]);
-    for (int e = threadIdx.x; e < nyt * hw; e += kThreads) {
-        const int yy = e / hw, j = e - yy * hw, y = y0 + yy;
+    const int plane = nyt * hw;
+    for (int e = threadIdx.x; e < nzl * plane; e += kThreads) {
+        const int zl = e / plane, r = e - zl * plane;
+        const int yy = r / hw, j = r - yy * hw;
+        const int z = z0 + zl, y = y0 + yy;
         const int p0 = (y + z) & 1;
         const int xa = 2 * j + p0, xb = 2 * j + (p0 ^ 1);
         const long rowi = (long)nx * (y + (long)ny * z);
         if (xb < nx) {
-            const V3<T> h1 = hval(yy, xb);
+            const V3<T> h1 = hval(zl, yy, xb);
             const long o = ((long)z * ny + y) * hw + j;
             hc[o] = h1.x;
             hc[cst + o] = h1.y;
@@ -1134,17 +1177,19 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
         if (xa < nx) {
             const long i = rowi + xa;
             const V3<T> mi = load_m(m, NN, i);
-            metro_update(in, m, xa, y, z, i, mi, hval(yy, xa), sweep, stage, bt);
+            metro_update(in, m, xa, y, z, i, mi, hval(zl, yy, xa), sweep, stage, bt);
         }
     }
 }
 
-// Second checkerboard colour of 2R real rows fused with the forward x pass
-// of the rows' final state (item as k_xinv_metro).  The rows are loaded into
-// LDS packed two real rows per complex row, colour-1 cells are updated from
-// the compact h and written to global m and to LDS, then the rows are
-// transformed and their half spectrum stored as k_xfwd_n does.
-template <typename T, int N, int R>
+// Second checkerboard colour of a tile of real rows fused with the forward x
+// pass of the rows' final state (items and LDS rows as k_xinv_metro).  The
+// rows are loaded into LDS packed two real rows per complex row, colour-1
+// cells are updated from the compact h and written to global m and to LDS,
+// then the rows are transformed and their half spectrum stored as k_xfwd_n
+// does (zf = 0), or transformed along z in registers and stored to the
+// xz-spectrum B as k_zfwd does (zf = 1).
+template <typename T, int N, int RZ>
 __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict__ I,
                                                          const Work *__restrict__ W,
                                                          T *__restrict__ mb,
@@ -1156,22 +1201,25 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
                                                          const cpx<T> *__restrict__ twb)
 {
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
-    cpx<T> *tw = s + 3 * R * N;
+    cpx<T> *tw = s + 3 * RZ * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
-    const int z = w.b, y0 = w.c;
+    const int z0 = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const int nzl = in.zf ? nz : 1;
+    const int R = RZ / nzl;
     const int nyt = min(2 * R, ny - y0);
     const long NN = in.N;
     T *m = mb + in.m_off;
     for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
-    for (int e = threadIdx.x; e < 3 * R * N; e += kThreads) {
-        const int c = e / (R * N), r = e - c * (R * N);
-        const int q = r / N, x = r - q * N;
+    for (int e = threadIdx.x; e < 3 * RZ * N; e += kThreads) {
+        const int c = e / (RZ * N), r = e - c * (RZ * N);
+        const int zr = r / N, x = r - zr * N;
+        const int zl = zr / R, q = zr - zl * R;
         T re = T(0), im = T(0);
         if (x < nx && 2 * q < nyt) {
-            const T *row = m + c * NN + (long)nx * ((y0 + 2 * q) + (long)ny * z);
+            const T *row = m + c * NN + (long)nx * ((y0 + 2 * q) + (long)ny * (z0 + zl));
             re = row[x];
             if (2 * q + 1 < nyt) im = row[(long)nx + x];
         }
@@ -1184,44 +1232,81 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
         const long cst = (long)hw * ny * nz;
         const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
         const T bt = T(beta[w.inst]);
-        for (int e = threadIdx.x; e < nyt * hw; e += kThreads) {
-            const int yy = e / hw, j = e - yy * hw, y = y0 + yy;
+        const int plane = nyt * hw;
+        for (int e = threadIdx.x; e < nzl * plane; e += kThreads) {
+            const int zl = e / plane, r = e - zl * plane;
+            const int yy = r / hw, j = r - yy * hw;
+            const int z = z0 + zl, y = y0 + yy;
             const int x = 2 * j + ((y + z + 1) & 1);
             if (x >= nx) continue;
-            const int o = (yy >> 1) * N + x;
+            const int o = (zl * R + (yy >> 1)) * N + x;
             const bool odd = yy & 1;
-            const V3<T> mi = odd ? V3<T>{s[o].y, s[R * N + o].y, s[2 * R * N + o].y}
-                                 : V3<T>{s[o].x, s[R * N + o].x, s[2 * R * N + o].x};
+            const V3<T> mi = odd ? V3<T>{s[o].y, s[RZ * N + o].y, s[2 * RZ * N + o].y}
+                                 : V3<T>{s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
             const long oh = ((long)z * ny + y) * hw + j;
             const V3<T> hi = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
             const long i = (long)nx * (y + (long)ny * z) + x;
             const V3<T> mn = metro_update(in, m, x, y, z, i, mi, hi, sweep, stage, bt);
             if (odd) {
                 s[o].y = mn.x;
-                s[R * N + o].y = mn.y;
-                s[2 * R * N + o].y = mn.z;
+                s[RZ * N + o].y = mn.y;
+                s[2 * RZ * N + o].y = mn.z;
             } else {
                 s[o].x = mn.x;
-                s[R * N + o].x = mn.y;
-                s[2 * R * N + o].x = mn.z;
+                s[RZ * N + o].x = mn.y;
+                s[2 * RZ * N + o].x = mn.z;
             }
         }
     }
     __syncthreads();
-    fft_fixed<T, N, 3 * R>(s, tw);
+    fft_fixed<T, N, 3 * RZ>(s, tw);
     const int npair = (nyt + 1) >> 1;
-    const long cstride = (long)H * nz * ny;
-    cpx<T> *dst = specb + in.a_off;
     const int per = H * npair;
-    for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
-        const int c = e / per, r = e - c * per;
-        const int k = r / npair, q = r - k * npair;
-        const cpx<T> *row = s + (c * R + q) * N;
-        const cpx<T> A = row[k];
-        const cpx<T> B = row[k ? N - k : 0];
-        cpx<T> *o = dst + c * cstride + ((long)k * nz + z) * ny + y0 + 2 * q;
-        o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
-        if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+    if (!in.zf) {
+        const long cstride = (long)H * nz * ny;
+        cpx<T> *dst = specb + in.a_off;
+        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+            const int c = e / per, r = e - c * per;
+            const int k = r / npair, q = r - k * npair;
+            const cpx<T> *row = s + (c * RZ + q) * N;
+            const cpx<T> A = row[k];
+            const cpx<T> B = row[k ? N - k : 0];
+            cpx<T> *o = dst + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
+            o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+            if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+        }
+    } else {
+        const int Pz = in.Pz;
+        const long cstride = (long)H * Pz * ny;
+        cpx<T> *dst = specb + in.b_off;
+        const cpx<T> *twz = twb + in.twz;
+        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+            const int c = e / per, r = e - c * per;
+            const int k = r / npair, q = r - k * npair;
+            const bool hasb = 2 * q + 1 < nyt;
+            cpx<T> xa[kZDirect], xb[kZDirect], oa[kZDirect], ob[kZDirect];
+            #pragma unroll
+            for (int zl = 0; zl < kZDirect; ++zl) {
+                xa[zl] = xb[zl] = {T(0), T(0)};
+                if (zl < nz) {
+                    const cpx<T> *row = s + (c * RZ + zl * R + q) * N;
+                    const cpx<T> A = row[k];
+                    const cpx<T> B = row[k ? N - k : 0];
+                    xa[zl] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+                    xb[zl] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+                }
+            }
+            zdft_regs<T, false>(xa, oa, nz, Pz, Pz, twz);
+            zdft_regs<T, false>(xb, ob, nz, Pz, Pz, twz);
+            cpx<T> *o = dst + c * cstride + (long)k * Pz * ny + y0 + 2 * q;
+            #pragma unroll
+            for (int kz = 0; kz < kZDirect; ++kz) {
+                if (kz < Pz) {
+                    o[(long)kz * ny] = oa[kz];
+                    if (hasb) o[(long)kz * ny + 1] = ob[kz];
+                }
+            }
+        }
     }
 }
 
