@@ -216,7 +216,10 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * inverse x pass runs with colour 0 and colour 1 with the next forward x
  * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
  * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
- * tile, 2048 FP32 / 1024 FP64 complex per component over the x length).
+ * tile, 2048 FP32 / 1024 FP64 complex per component over the x length),
+ * "zfold" (1: fused tiles of instances with a short padded z axis, Pz <= 8,
+ * span all layers and do the z transform in registers, so sweeps skip the
+ * separate z passes; default 0).
  * Tile and fusion options only regroup independent work: results are
  * bit-identical.
  * Unknown keys return MMC_EINVAL. */

@@ -122,6 +122,7 @@ struct mmc_ctx {
     bool metro_tiled_ok = false;      // every instance admits the tiled kernel
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
     bool fuse_ok = false;             // every instance admits them
+    bool zfold = false;               // fused tiles of short-z instances fold z in
     // per instance (tyx, ncol, tyf): widest tiles that fit, and the tiles
     // chosen for the launch sizes (the autotuner's starting points)
     std::vector<std::array<int, 3>> tile_full, tile_default;
@@ -386,8 +387,9 @@ bool xfixed(const mmc_ctx *c, const Inst &in)
 // Complex rows per component of a full fused x + Metropolis tile of
 // instance `in` (0 when the instance cannot take the fused passes: no demag,
 // three colours, an x length without a fixed plan, or LDS too small for one
-// row pair).  Sets in.zf when a tile can span every layer and fold the z
-// transform in (Pz <= kZDirect and the layers divide the tile).
+// row pair).  With the "zfold" option, sets in.zf when a tile can span every
+// layer and fold the z transform in (Pz <= kZDirect and the layers divide
+// the tile).
 int fused_full(const mmc_ctx *c, Inst &in)
 {
     in.zf = 0;
@@ -396,7 +398,7 @@ int fused_full(const mmc_ctx *c, Inst &in)
     const int r = fused_rows(c->tsize == 4 ? fused_rows_base<float>() : fused_rows_base<double>(),
                              in.Px);
     const bool fits = (3 * (size_t)r + 1) * in.Px * cs <= c->lds_max;
-    if (fits && in.Pz > 1 && in.Pz <= kZDirect && r % in.nz == 0) {
+    if (c->zfold && fits && in.Pz > 1 && in.Pz <= kZDirect && r % in.nz == 0) {
         in.zf = 1;
         return r;
     }
@@ -533,7 +535,8 @@ void build_lists(mmc_ctx *c)
     }
     c->zls.lds = c->zl.lds;
     for (auto &kv : fitems) {
-        xcd_group(kv.second, line_share(2 * kv.first.second, cs));
+        const Inst &i0 = c->inst[kv.second.front().inst];
+        xcd_group(kv.second, line_share(2 * kv.first.second / (i0.zf ? i0.nz : 1), cs));
         Segment sg = fclass[kv.first];
         sg.begin = c->fl.n();
         sg.count = (int)kv.second.size();
@@ -1194,7 +1197,22 @@ struct Schedule {
     const char *name;
     bool fuse, tiled;
     int tiles;          // 0 chosen for the launch sizes, 1 minimal, 2 widest
+    bool zfold;         // fused tiles of short-z instances fold z in
 };
+
+// Fused tiles that fold z in where an instance admits them (zfold), else
+// the tiles already chosen.
+void set_zfold(mmc_ctx *c, bool on)
+{
+    c->zfold = on;
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        Inst &in = c->inst[i];
+        if (!c->fuse_ok) { in.zf = 0; continue; }
+        const int keep = in.zf ? c->tile_default[i][2] : in.tyf;
+        const int r = fused_full(c, in);
+        in.tyf = in.zf ? r : keep;
+    }
+}
 
 int apply_schedule(mmc_ctx *c, const Schedule &s)
 {
@@ -1212,6 +1230,8 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
     }
     c->fuse = s.fuse;
     c->metro_tiled = s.tiled;
+    for (auto &in : c->inst) in.zf = 0;
+    set_zfold(c, s.zfold);
     return retile(c, 0, 0);
 }
 
@@ -1246,14 +1266,26 @@ template <typename T>
 int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
 {
     static const Schedule all[] = {
-        {"fused, launch-sized tiles", true, false, 0},
-        {"fused, minimal tiles", true, false, 1},
-        {"fused, widest tiles", true, false, 2},
-        {"separate, launch-sized tiles", false, false, 0},
-        {"separate, minimal tiles", false, false, 1},
-        {"separate, widest tiles", false, false, 2},
-        {"tiled two-colour Metropolis", false, true, 0},
+        {"fused, launch-sized tiles", true, false, 0, false},
+        {"fused, minimal tiles", true, false, 1, false},
+        {"fused, widest tiles", true, false, 2, false},
+        {"fused, z folded", true, false, 0, true},
+        {"separate, launch-sized tiles", false, false, 0, false},
+        {"separate, minimal tiles", false, false, 1, false},
+        {"separate, widest tiles", false, false, 2, false},
+        {"tiled two-colour Metropolis", false, true, 0, false},
     };
+    bool any_zf = false;
+    {
+        const bool z = c->zfold;
+        c->zfold = true;
+        for (auto &in : c->inst) {
+            Inst t = in;
+            fused_full(c, t);
+            any_zf = any_zf || t.zf;
+        }
+        c->zfold = z;
+    }
     void *save = nullptr;
     const size_t mbytes = std::max<size_t>(1, c->m_elems) * c->tsize;
     HIPCHK(c, hipMalloc(&save, mbytes));
@@ -1266,7 +1298,8 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
     char line[160];
     for (int k = 0; k < (int)(sizeof(all) / sizeof(all[0])) && !rc; ++k) {
         const Schedule &s = all[k];
-        if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok)) continue;
+        if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf))
+            continue;
         rc = apply_schedule(c, s);
         double us = 0;
         if (!rc) rc = time_sweeps<T>(c, sweeps, &us);
@@ -1706,6 +1739,10 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         if (value < 1) return fail(c, MMC_EINVAL, "fuse_rows must be >= 1");
         for (auto &in : c->inst)
             if (in.tyf) in.tyf = (value > 1 || in.zf) ? fused_full(c, in) : 1;
+        return retile(c, 0, 0);
+    }
+    if (k == "zfold") {
+        set_zfold(c, value != 0);
         return retile(c, 0, 0);
     }
     if (k == "fuse") {

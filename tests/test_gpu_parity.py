@@ -235,7 +235,7 @@ def test_tiled_metropolis_matches_per_colour(mmc, geo, precision):
                                  dict(nx=64, ny=16, nz=4, pbc_z=1),
                                  dict(nx=256, ny=50, nz=1, pbc_x=3),
                                  dict(nx=128, ny=37, nz=2),
-                                 dict(nx=64, ny=10, nz=2, pbc_x=1, pbc_y=1)],
+                                 dict(nx=128, ny=10, nz=2, pbc_x=1, pbc_y=1)],
                          ids=["open", "pbc", "odd_open", "3d_open", "3d_pbcz", "wide_pbc",
                               "zfold_open", "zfold_pbc"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
@@ -246,13 +246,15 @@ def test_fused_sweeps_match_separate(mmc, geo, precision):
     launches."""
     kw = film(**geo)
     out = []
-    for fuse, rows, graph in ((0, 1, 0), (1, 1, 0), (1, 2, 0), (1, 2, 1)):
+    for fuse, rows, graph, zfold in ((0, 1, 0, 0), (1, 1, 0, 0), (1, 2, 0, 0), (1, 2, 1, 0),
+                                     (1, 2, 1, 1)):
         c = mmc.Context(mmc.make_params(**kw), precision)
         c.build_tensor()
         c.init_random_m()
         c.set_temperature(650.0)
         c.set_option("fuse", fuse)
         c.set_option("fuse_rows", rows)
+        c.set_option("zfold", zfold)
         if graph:
             c.set_option("graph", 1)
             c.set_option("graph_sweeps", 3)
@@ -413,15 +415,17 @@ def test_autotune_keeps_the_chain(mmc, geo):
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_fused_tiles_fold_short_z(mmc, precision):
-    """Two-layer films take fused tiles that span both layers and transform z
-    in registers: the fused passes move the xz spectrum B (Pz planes), and
-    the sweep has no separate z passes."""
+    """With "zfold", two-layer films take fused tiles that span both layers
+    and transform z in registers: the fused passes move the xz spectrum B (Pz
+    planes), and the sweep has no separate z passes."""
     b = 8 if precision == 0 else 4
     c = mmc.Context(mmc.make_params(**film(nx=128, ny=40, nz=2)), precision)
     Px, Py, Pz = c.P()
     assert Pz == 3
     H = Px // 2 + 1
     N = 128 * 40 * 2
+    assert c.kernel_bytes("xinv_metro") == 3 * H * 2 * 40 * 2 * b + 6 * N * b
+    c.set_option("zfold", 1)
     assert c.kernel_bytes("xinv_metro") == 3 * H * Pz * 40 * 2 * b + 6 * N * b
     c.build_tensor()
     c.init_random_m()
