@@ -164,7 +164,21 @@ def copy_calibration(nbytes, times):
     hip.hipFree(b)
 
 
-def roofline(kern):
+def measured_traffic(workload, kind):
+    """HBM bytes per launch of `kind` on `workload` from the committed
+    hardware-counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a device copy of known size), or
+    None when no counter run covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["workloads"][workload][kind]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(kern, workload=None):
     """Dominant kernel (largest device time) and its achieved bandwidth:
     algorithmic bytes of its timed launches over their summed duration."""
     if not kern:
@@ -173,7 +187,8 @@ def roofline(kern):
     k = kern[dom]
     gbs = k["total_bytes"] / (k["total_ms"] * 1e-3) / 1e9
     return dom, dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                     frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                     frac=round(gbs / HBM_PEAK_GBS, 4),
+                     traffic=measured_traffic(workload, dom) if workload else None,
                      kernel=dom, avg_us=round(k["avg_us"], 3),
                      bytes_per_launch=k["bytes_per_launch"])
 
@@ -236,7 +251,7 @@ def main():
     log(f"[bench] {a.workload} FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
     r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps,
                   default_stream=a.default_stream, autotune=a.autotune)
-    dom, roof = roofline(r32["kernels"])
+    dom, roof = roofline(r32["kernels"], a.workload)
     extra = {}
     if not a.no_extra:
         log("[bench] same film, plain FP64")
@@ -268,7 +283,7 @@ def main():
         kw4 = film1024x2()
         log("[bench] 1024x1024x2 PBC10 FP32-bulk")
         r4 = run_gpu(mod, kw4, mod.FP32, 200, 20, autotune=a.autotune)
-        d4, roof4 = roofline(r4["kernels"])
+        d4, roof4 = roofline(r4["kernels"], "film1024x2")
         extra["film1024x2_attempts_per_s"] = r4["attempts_per_s"]
         extra["film1024x2_ms_per_sweep"] = r4["ms_per_sweep"]
         extra["film1024x2_tensor_build_s"] = r4["tensor_build_s"]
