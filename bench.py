@@ -58,6 +58,45 @@ def film1024x2(**kw):
     return d
 
 
+def film128x16(**kw):
+    d = dict(nx=128, ny=128, nz=16, dx=1e-9, dy=1e-9, dz=1e-9, pbc_x=0, pbc_y=0,
+             pbc_z=0, ms=1.6e6, aex=13e-12, dind=3e-3, anis_ux=0.0, anis_uy=0.0,
+             anis_uz=1.0, ku1=8e5, bext_x=0.0, bext_y=0.0, bext_z=0.2, cal_demag=1,
+             seed=128)
+    d.update(kw)
+    return d
+
+
+def fp32_errors(mod, kw, ctx32):
+    """FP32-bulk error against the FP64 path (itself equal to the FP64 oracle
+    within 1e-12) on the same states: the seeded start state and the FP32
+    context's current state.  Norm-wise relative errors of H_eff (all terms),
+    the total energy and every cell's trial-move dE."""
+    import numpy as np
+    c64 = mod.Context(mod.make_params(**kw), mod.FP64)
+    c64.build_tensor()
+    c32 = mod.Context(mod.make_params(**kw), mod.FP32)
+    c32.build_tensor()
+    out = {}
+    for name, src in (("start", None), ("after_timed_sweeps", ctx32)):
+        if src is None:
+            c32.init_random_m()
+        else:
+            c32.set_m(src.m())
+        m = c32.m()
+        c64.set_m(m)
+        c32.set_m(m)
+        h32, h64 = c32.field(), c64.field()
+        e32, e64 = c32.energy()["total"], c64.energy()["total"]
+        d32, d64 = c32.trial_delta_e(7, 3), c64.trial_delta_e(7, 3)
+        out[name] = dict(h_eff_rel=float(np.abs(h32 - h64).max() / np.abs(h64).max()),
+                         energy_rel=float(abs(e32 - e64) / abs(e64)),
+                         delta_e_rel=float(np.abs(d32 - d64).max() / np.abs(d64).max()))
+    c64.close()
+    c32.close()
+    return out
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -280,6 +319,15 @@ def main():
         extra["batch64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in rb["kernels"].items()}
         extra["batched_fp32_over_single_fp64"] = rb["attempts_per_s"] / r0["attempts_per_s"]
         rb["ctx"].close()
+        log("[bench] 128x128x16 open 3-D film FP32-bulk")
+        r2 = run_gpu(mod, film128x16(), mod.FP32, 200, 20, autotune=a.autotune)
+        extra["film128x16_attempts_per_s"] = r2["attempts_per_s"]
+        extra["film128x16_ms_per_sweep"] = r2["ms_per_sweep"]
+        extra["film128x16_schedule"] = r2["schedule"]
+        extra["film128x16_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r2["kernels"].items()}
+        r2["ctx"].close()
+        log("[bench] FP32-bulk error against FP64 on the 256x256 film")
+        extra["fp32_error_vs_fp64"] = fp32_errors(mod, kw, r32["ctx"])
         kw4 = film1024x2()
         log("[bench] 1024x1024x2 PBC10 FP32-bulk")
         r4 = run_gpu(mod, kw4, mod.FP32, 200, 20, autotune=a.autotune)

@@ -23,9 +23,10 @@
  * restated from the physics their signatures fix (src/effective_H.h:25-43,
  * src/mc_sampling.h:27-41, src/demag_mirror.h:18-24):
  *   exchange / interfacial DMI / uniaxial anisotropy / Zeeman field and
- *   energy density, the single-cell energy change, the mirror (self) tensor
- *   and the Metropolis update.  Parity for those is pinned in tests/ by
- *   closed forms, invariants and statistics, not by reference output.
+ *   energy density, the single-cell energy change (and its value for every
+ *   cell's trial move, orc_trial_delta_e), the mirror (self) tensor and the
+ *   Metropolis update.  Parity for those is pinned in tests/ by closed
+ *   forms, invariants and statistics, not by reference output.
  */
 #include "mmc_oracle.h"
 
