@@ -136,6 +136,12 @@ int mmc_field(mmc_ctx *ctx, int inst, int terms, double *hx, double *hy,
 int mmc_energy(mmc_ctx *ctx, int inst, double *E_total, double e_terms[5],
                double m_avg[3]);
 
+/* Energy density [J/m^3] of every real cell (N doubles), the selected terms
+ * summed.  Replaces calEnergyKernel's per-cell output d_energy
+ * (src/effective_H.h:35-43), which the reference dumps as E_density.ovf
+ * under DEBUG_H_FIELD (src/main.cu:1050-1064). */
+int mmc_energy_density(mmc_ctx *ctx, int inst, int terms, double *e);
+
 /* Energy change [J] of every cell's Metropolis trial move for (stage,
  * sweep) — Philox counter (cell, sweep, stage, 0) — against the current
  * state and its demag field, the moves not applied (N doubles).  The dE

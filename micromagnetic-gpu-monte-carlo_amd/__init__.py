@@ -108,7 +108,7 @@ EXPORTS = (
     "mmc_create", "mmc_batch_create", "mmc_destroy", "mmc_last_error",
     "mmc_dims", "mmc_count", "mmc_build_tensor", "mmc_set_tensor",
     "mmc_get_tensor", "mmc_mirror_tensor", "mmc_set_m", "mmc_get_m",
-    "mmc_init_random_m", "mmc_field", "mmc_trial_delta_e", "mmc_energy",
+    "mmc_init_random_m", "mmc_field", "mmc_energy_density", "mmc_trial_delta_e", "mmc_energy",
     "mmc_set_temperature",
     "mmc_sweeps", "mmc_autotune", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
     "mmc_kernel_stats", "mmc_kernel_bytes",
@@ -168,6 +168,7 @@ def lib() -> C.CDLL:
     L.mmc_init_random_m.argtypes = [vp, C.c_int]
     L.mmc_field.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
     L.mmc_trial_delta_e.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, dp]
+    L.mmc_energy_density.argtypes = [vp, C.c_int, C.c_int, dp]
     L.mmc_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
     L.mmc_energy.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mmc_set_temperature.argtypes = [vp, C.c_int, C.c_double]
@@ -335,6 +336,13 @@ class Context:
         out = np.empty((3, nz, ny, nx))
         self._check(lib().mmc_field(self._ptr, inst, terms, _dp(out[0]),
                                     _dp(out[1]), _dp(out[2])))
+        return out
+
+    def energy_density(self, terms: int = ALL, inst: int = 0) -> np.ndarray:
+        """Energy density [J/m^3] of the selected terms; shape (nz, ny, nx)."""
+        nx, ny, nz = self.n(inst)
+        out = np.empty((nz, ny, nx))
+        self._check(lib().mmc_energy_density(self._ptr, inst, terms, _dp(out)))
         return out
 
     def trial_delta_e(self, stage: int, sweep: int, inst: int = 0) -> np.ndarray:

@@ -1543,6 +1543,28 @@ int mmc_field(mmc_ctx *c, int inst, int terms, double *hx, double *hy, double *h
     return MMC_OK;
 }
 
+int mmc_energy_density(mmc_ctx *c, int inst, int terms, double *e)
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!e) return fail(c, MMC_EINVAL, "null output");
+    const Inst &in = c->inst[inst];
+    if ((terms & MMC_DEMAG) && in.cal_demag) {
+        rc = c->tsize == 4 ? demag_chain<float>(c) : demag_chain<double>(c);
+        if (rc) return rc;
+    }
+    const int b0 = c->cl.first[inst], b1 = c->cl.first[inst + 1];
+    hipError_t err = c->tsize == 4
+        ? launch_energy_density<float>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0,
+                                       (const float *)c->d_m, (const float *)c->d_h, terms, c->d_H)
+        : launch_energy_density<double>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0,
+                                        (const double *)c->d_m, (const double *)c->d_h, terms, c->d_H);
+    HIPCHK(c, err);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(e, c->d_H, sizeof(double) * in.N, hipMemcpyDeviceToHost));
+    return MMC_OK;
+}
+
 int mmc_trial_delta_e(mmc_ctx *c, int inst, uint32_t stage, uint32_t sweep, double *dE)
 {
     int rc = check_inst(c, inst);

@@ -156,6 +156,10 @@ template <typename T>
 hipError_t launch_field(hipStream_t st, const Inst *I, const Work *W, int n,
                         const T *m, const T *h, int terms, double *H);
 template <typename T>
+hipError_t launch_energy_density(hipStream_t st, const Inst *I, const Work *W,
+                                 int n, const T *m, const T *h, int terms,
+                                 double *out);
+template <typename T>
 hipError_t launch_delta_e(hipStream_t st, const Inst *I, const Work *W, int n,
                           const T *m, const T *h, uint32_t stage,
                           uint32_t sweep, double *out);

@@ -18,6 +18,7 @@
 //                   block partials, fixed-order final sum (src/main.cu:945-947)
 //   k_field         effective field in A/m (effectiveHKernel,
 //                   src/effective_H.h:25-33)
+//   k_energy_density per-cell energy density of selected terms
 //   k_delta_e       energy change of every cell's trial move (diagnostic)
 //   k_init          Philox uniform-on-sphere start state
 #include "fft_fixed.h"
@@ -1356,6 +1357,37 @@ __device__ __forceinline__ void block_reduce(DD *v)
 // Per-cell energy densities (units mu0 Ms^2):
 //   e_ex = -m.(ex - cnt m)/2, e_dmi = -m.dmi/2, e_anis = -kap (m.u)^2,
 //   e_zee = -m.b, e_demag = -m.h/2;  plus m itself.
+// Energy terms of cell i (units mu0 Ms^2 V): e[0] exchange, e[1] DMI,
+// e[2] anisotropy, e[3] Zeeman, e[4] demag (h of the current state); mi
+// returns the cell's state.
+template <typename T>
+__device__ __forceinline__ void cell_energy(const Inst &in, const T *mb, const T *hb, long i,
+                                            int use_demag, double e[5], V3<T> &mi)
+{
+    const int nx = in.nx, ny = in.ny;
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const T *m = mb + in.m_off;
+    const long N = in.N;
+    mi = load_m(m, N, i);
+    V3<T> ex, dmi;
+    T cnt;
+    stencil(in, m, x, y, z, ex, dmi, cnt);
+    const T hex_x = ex.x - cnt * mi.x, hex_y = ex.y - cnt * mi.y, hex_z = ex.z - cnt * mi.z;
+    const T mu = mi.x * T(in.u[0]) + mi.y * T(in.u[1]) + mi.z * T(in.u[2]);
+    e[0] = double(T(-0.5) * (mi.x * hex_x + mi.y * hex_y + mi.z * hex_z));
+    e[1] = double(T(-0.5) * (mi.x * dmi.x + mi.y * dmi.y + mi.z * dmi.z));
+    e[2] = double(-T(in.kap) * mu * mu);
+    e[3] = double(-(mi.x * T(in.b[0]) + mi.y * T(in.b[1]) + mi.z * T(in.b[2])));
+    e[4] = 0.0;
+    if (use_demag) {
+        const T *h = hb + in.h_off;
+        e[4] = double(T(-0.5) * (mi.x * h[i] + mi.y * h[N + i] + mi.z * h[2 * N + i]));
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_energy(const Inst *__restrict__ I,
                                                      const Work *__restrict__ W,
@@ -1371,27 +1403,11 @@ __global__ __launch_bounds__(kThreads) void k_energy(const Inst *__restrict__ I,
     #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = {0.0, 0.0};
     if (i < in.N) {
-        const int nx = in.nx, ny = in.ny;
-        const int x = (int)(i % nx);
-        const long r = i / nx;
-        const int y = (int)(r % ny);
-        const int z = (int)(r / ny);
-        const T *m = mb + in.m_off;
-        const long N = in.N;
-        const V3<T> mi = load_m(m, N, i);
-        V3<T> ex, dmi;
-        T cnt;
-        stencil(in, m, x, y, z, ex, dmi, cnt);
-        const T hex_x = ex.x - cnt * mi.x, hex_y = ex.y - cnt * mi.y, hex_z = ex.z - cnt * mi.z;
-        const T mu = mi.x * T(in.u[0]) + mi.y * T(in.u[1]) + mi.z * T(in.u[2]);
-        v[0].hi = double(T(-0.5) * (mi.x * hex_x + mi.y * hex_y + mi.z * hex_z));
-        v[1].hi = double(T(-0.5) * (mi.x * dmi.x + mi.y * dmi.y + mi.z * dmi.z));
-        v[2].hi = double(-T(in.kap) * mu * mu);
-        v[3].hi = double(-(mi.x * T(in.b[0]) + mi.y * T(in.b[1]) + mi.z * T(in.b[2])));
-        if (use_demag) {
-            const T *h = hb + in.h_off;
-            v[4].hi = double(T(-0.5) * (mi.x * h[i] + mi.y * h[N + i] + mi.z * h[2 * N + i]));
-        }
+        double e[5];
+        V3<T> mi;
+        cell_energy(in, mb, hb, i, use_demag, e, mi);
+        #pragma unroll
+        for (int q = 0; q < 5; ++q) v[q].hi = e[q];
         v[5].hi = double(mi.x);
         v[6].hi = double(mi.y);
         v[7].hi = double(mi.z);
@@ -1463,6 +1479,29 @@ __global__ __launch_bounds__(kThreads) void k_field(const Inst *__restrict__ I,
     H[i] = in.ms * double(hx);
     H[N + i] = in.ms * double(hy);
     H[2 * N + i] = in.ms * double(hz);
+}
+
+// Energy density [J/m^3] of every cell, the selected terms (bits 1 exch,
+// 2 dmi, 4 anis, 8 zeeman, 16 demag) summed in that order.  Output N doubles.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_energy_density(const Inst *__restrict__ I,
+                                                             const Work *__restrict__ W,
+                                                             const T *__restrict__ mb,
+                                                             const T *__restrict__ hb, int terms,
+                                                             double *__restrict__ out)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const long i = (long)w.a + threadIdx.x;
+    if (i >= in.N) return;
+    double e[5];
+    V3<T> mi;
+    cell_energy(in, mb, hb, i, in.cal_demag, e, mi);
+    double t = 0.0;
+    #pragma unroll
+    for (int q = 0; q < 5; ++q)
+        if (terms & (1 << q)) t += e[q];
+    out[i] = t * (in.e_unit / (in.d[0] * in.d[1] * in.d[2]));
 }
 
 // Energy change [J] of every cell's trial move (Philox counter (cell, sweep,
@@ -1739,6 +1778,15 @@ hipError_t launch_field(hipStream_t st, const Inst *I, const Work *W, int n, con
 }
 
 template <typename T>
+hipError_t launch_energy_density(hipStream_t st, const Inst *I, const Work *W, int n, const T *m,
+                                 const T *h, int terms, double *out)
+{
+    if (n <= 0) return hipSuccess;
+    MMC_LAUNCH(k_energy_density<T>, dim3(n), dim3(kThreads), 0, st, I, W, m, h, terms, out);
+    return post();
+}
+
+template <typename T>
 hipError_t launch_delta_e(hipStream_t st, const Inst *I, const Work *W, int n, const T *m,
                           const T *h, uint32_t stage, uint32_t sweep, double *out)
 {
@@ -1846,6 +1894,8 @@ hipError_t set_lds_limit(size_t bytes)
                                          const T *, const T *, double *, int);                \
     template hipError_t launch_field<T>(hipStream_t, const Inst *, const Work *, int,         \
                                         const T *, const T *, int, double *);                 \
+    template hipError_t launch_energy_density<T>(hipStream_t, const Inst *, const Work *, int, \
+                                                 const T *, const T *, int, double *);       \
     template hipError_t launch_delta_e<T>(hipStream_t, const Inst *, const Work *, int,       \
                                           const T *, const T *, uint32_t, uint32_t, double *); \
     template hipError_t launch_init<T>(hipStream_t, const Inst *, const Work *, int, T *);

@@ -1,12 +1,16 @@
 // mmc_main.cpp -- command-line annealer, the drop-in for the reference's
 // executable (src/main.cu:90-1266):
 //
-//   mmc_run input.ini [-gpu=N] [-tensorcache="dir/"]
+//   mmc_run input.ini [-gpu=N] [-tensorcache="dir/"] [-debugfields]
 //
 // Reads the reference's ini keys, writes <ini-dir>/<ini-stem>.output/ with
 // energy.txt ("%f %ld %g": T, sweep, E[J]) and OVF snapshots
 // "%06ld_%.2fK_%ld.ovf" (counter, T, sweep), loads/stores the tensor cache,
-// and runs the temperature-stage loop with Ctrl-C handling.
+// and runs the temperature-stage loop with Ctrl-C handling.  -debugfields
+// also writes, at every snapshot, the per-term fields B_demag, B_anis,
+// B_exch, B_dmi, B_zeeman, B_effective [T] and E_density [J/m^3, in all three
+// components] as OVF files of those names (overwritten), as the reference
+// does under its compile-time DEBUG_H_FIELD switch (src/main.cu:965-1064).
 #include "mmc.h"
 #include "mmc_io.h"
 
@@ -30,7 +34,8 @@ static void on_sigint(int) { g_stop = 1; }
 static void usage()
 {
     std::printf("need input file. \n");
-    std::printf("arg format:  input_cfg_file.ini  -gpu=0   -tensorcache=\"/mnt/cachepath/\" \n");
+    std::printf("arg format:  input_cfg_file.ini  -gpu=0   -tensorcache=\"/mnt/cachepath/\" "
+                "[-debugfields]\n");
 }
 
 static void echo_config(const mmc_config &c)   // cout_input_ini, src/parse_input_ini.h:49-114
@@ -58,6 +63,32 @@ static void echo_config(const mmc_config &c)   // cout_input_ini, src/parse_inpu
 
 static bool due(long s, long period) { return period > 0 && s % period == 0; }
 
+// Per-term fields [T] and the energy density of instance 0 as OVF files in
+// `dir` (the reference's DEBUG_H_FIELD dumps, src/main.cu:965-1064).
+static int write_debug_fields(mmc_ctx *ctx, const fs::path &dir, const mmc_ovf_info &info,
+                              long N, int fmt)
+{
+    const double mu0 = 4e-7 * 3.14159265358;          // src/main.cu:244
+    std::vector<double> hx(N), hy(N), hz(N);
+    static const struct { const char *name; int terms; } f[] = {
+        {"B_demag.ovf", MMC_DEMAG}, {"B_anis.ovf", MMC_ANIS}, {"B_exch.ovf", MMC_EXCH},
+        {"B_dmi.ovf", MMC_DMI}, {"B_zeeman.ovf", MMC_ZEEMAN}, {"B_effective.ovf", MMC_ALL}};
+    for (const auto &t : f) {
+        int rc = mmc_field(ctx, 0, t.terms, hx.data(), hy.data(), hz.data());
+        if (rc != MMC_OK) return rc;
+        for (long i = 0; i < N; ++i) { hx[i] *= mu0; hy[i] *= mu0; hz[i] *= mu0; }
+        if (mmc_ovf_write((dir / t.name).c_str(), &info, hx.data(), hy.data(), hz.data(), fmt,
+                          "B") != MMC_OK)
+            std::fprintf(stderr, "cannot write %s\n", t.name);
+    }
+    int rc = mmc_energy_density(ctx, 0, MMC_ALL, hx.data());
+    if (rc != MMC_OK) return rc;
+    if (mmc_ovf_write((dir / "E_density.ovf").c_str(), &info, hx.data(), hx.data(), hx.data(), fmt,
+                      "E_density") != MMC_OK)
+        std::fprintf(stderr, "cannot write E_density.ovf\n");
+    return MMC_OK;
+}
+
 // next sweep index >= s+1 at which any output falls due (or `end`)
 static long next_due(long s, long end, const long *periods, int n)
 {
@@ -77,6 +108,7 @@ int main(int argc, char **argv)
     if (argc == 1) { usage(); return 0; }
     std::string ini, cache_dir;
     int gpu = -1;
+    bool debug_fields = false;
     for (int i = 1; i < argc; ++i) {                 // src/GPU_selector.h:11-68
         std::string a = argv[i];
         if (a.rfind("-gpu=", 0) == 0) gpu = std::atoi(a.c_str() + 5);
@@ -84,7 +116,8 @@ int main(int argc, char **argv)
             cache_dir = a.substr(13);
             if (cache_dir.size() >= 2 && cache_dir.front() == '"' && cache_dir.back() == '"')
                 cache_dir = cache_dir.substr(1, cache_dir.size() - 2);
-        } else if (!a.empty() && a[0] != '-' && ini.empty()) ini = a;
+        } else if (a == "-debugfields") debug_fields = true;
+        else if (!a.empty() && a[0] != '-' && ini.empty()) ini = a;
     }
     if (ini.empty()) { usage(); return 0; }
     if (gpu >= 0 && hipSetDevice(gpu) != hipSuccess) {
@@ -184,8 +217,17 @@ int main(int argc, char **argv)
     auto t0 = std::chrono::steady_clock::now();
     rc = mmc_build_tensor(ctx, cache_dir.empty() ? nullptr : cache_dir.c_str());
     if (rc != MMC_OK) { std::fprintf(stderr, "%s\n", mmc_last_error(ctx)); mmc_destroy(ctx); return 1; }
-    std::printf("Done (%.3f s).\nStart...\n",
+    std::printf("Done (%.3f s).\n",
                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    // replay captured sweeps and keep the fastest sweep schedule (every
+    // schedule gives the same Markov chain)
+    if ((rc = mmc_set_option(ctx, "graph", 1)) == MMC_OK) {
+        char rep[2048];
+        rc = mmc_autotune(ctx, 10, rep, sizeof rep);
+        if (rc == MMC_OK) std::printf("%s", rep);
+    }
+    if (rc != MMC_OK) { std::fprintf(stderr, "%s\n", mmc_last_error(ctx)); mmc_destroy(ctx); return 1; }
+    std::printf("Start...\n");
 
     // temperature-stage loop, src/main.cu:801-1139
     double T = cfg.Temperature_start;
@@ -214,6 +256,9 @@ int main(int argc, char **argv)
                                   cfg.ovf_format, "m") != MMC_OK)
                     std::fprintf(stderr, "cannot write %s\n", name);
                 ++file_count;
+                if (debug_fields &&
+                    (rc = write_debug_fields(ctx, out_dir, info, N, cfg.ovf_format)) != MMC_OK)
+                    break;
             }
             if (due(s, cfg.energy_output_period)) {
                 double E = 0;

@@ -72,3 +72,39 @@ def test_cli_anneal_outputs_and_resume(mmc, tmp_path):
     assert len(lines2) == 1
     # the resumed start state is the snapshot: same energy as logged then
     assert lines2[0].split()[2] == E
+
+
+@pytest.mark.gpu
+def test_cli_debug_fields(mmc, tmp_path):
+    """-debugfields writes the per-term fields [T] and the energy density of
+    each snapshot state (the reference's DEBUG_H_FIELD dumps): they equal the
+    library's field and energy density of the snapshot's magnetisation."""
+    f = _ini(tmp_path, name="dbg", nx=24, ny=20, pbc_x=2, Temperature_start=500,
+             Temperature_end=400, Temperature_step=0.7, circle_per_stage=12,
+             terminal_output_period=100, energy_output_period=100, ms_output_period=12,
+             precision="fp64")
+    r = subprocess.run([str(CLI), str(f), "-gpu=0", "-debugfields"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = tmp_path / "dbg.output"
+    names = {p.name for p in out.glob("*.ovf")}
+    for n in ("B_demag", "B_anis", "B_exch", "B_dmi", "B_zeeman", "B_effective", "E_density"):
+        assert f"{n}.ovf" in names
+    snap = sorted(n for n in names if n[0].isdigit())[-1]
+    info, m = mmc.ovf_read(str(out / snap))
+    cfg = mmc.ini_parse(str(f))
+    c = mmc.Context(mmc.make_params(nx=24, ny=20, nz=1, pbc_x=cfg.pbc_x, pbc_y=cfg.pbc_y,
+                                    pbc_z=cfg.pbc_z, dx=cfg.dx, dy=cfg.dy,
+                                    dz=cfg.dz, ms=cfg.ms, aex=cfg.Aex, dind=cfg.Dind,
+                                    anis_ux=cfg.anisUx, anis_uy=cfg.anisUy, anis_uz=cfg.anisUz,
+                                    ku1=cfg.Ku1, bext_x=cfg.Bextx, bext_y=cfg.Bexty,
+                                    bext_z=cfg.Bextz, seed=cfg.randomseed), 0)
+    c.build_tensor()
+    c.set_m(m)
+    mu0 = 4e-7 * 3.14159265358
+    _, B = mmc.ovf_read(str(out / "B_effective.ovf"))
+    H = c.field(31)
+    assert np.abs(B - mu0 * H).max() <= 1e-12 * np.abs(mu0 * H).max()
+    _, Ed = mmc.ovf_read(str(out / "E_density.ovf"))
+    e = c.energy_density(31)
+    assert np.abs(Ed[2] - e).max() <= 1e-12 * np.abs(e).max()

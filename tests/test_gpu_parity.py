@@ -436,3 +436,19 @@ def test_fused_tiles_fold_short_z(mmc, precision):
     assert n == 1          # only the prologue's z pass of the xz spectrum
     n, ms, by = c.kernel_stats("zinv", reset=True)
     assert n == 0
+
+
+@pytest.mark.parametrize("name,geo", CASES, ids=[c[0] for c in CASES])
+def test_energy_density_fp64(orc, mmc, name, geo):
+    """Per-cell energy density of each term and of all terms matches the
+    oracle's within 1e-12 of its largest value, and sums to the energy."""
+    kw = film(**geo) if "3d" not in name else dict(FILM3D, **geo)
+    o, c = pair(orc, mmc, **kw)
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 17)
+    o.set_m(m)
+    c.set_m(m)
+    for _, bit in TERMS:
+        assert rel_inf(c.energy_density(bit), o.energy_density(bit)) <= 1e-12
+    V = kw["dx"] * kw["dy"] * kw["dz"]
+    E = c.energy()["total"]
+    assert abs(c.energy_density(31).sum() * V - E) <= 1e-10 * abs(E)
