@@ -165,7 +165,9 @@ int mmc_sweeps(mmc_ctx *ctx, uint32_t stage, uint32_t sweep0, uint32_t count);
 
 /* Time every admissible sweep schedule — fused x-pass + Metropolis kernels
  * or separate launches or the tiled two-colour Metropolis kernel, each with
- * launch-sized, minimal and widest FFT tiles — over sweeps_per_trial sweeps
+ * launch-sized, minimal and widest FFT tiles, z folding, and (for batches
+ * with several lengths) concurrent per-length launches — over
+ * sweeps_per_trial sweeps
  * (twice: once untimed) and keep the fastest.  Every schedule gives the
  * bit-identical Markov chain; the magnetisation is restored afterwards.  A
  * text table of the timings (and the choice) is written to report when it
@@ -223,6 +225,9 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
  * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
  * tile, 2048 FP32 / 1024 FP64 complex per component over the x length),
+ * "concurrent" (1: the per-length launches of a pass over a batch run on
+ * side streams forked from and joined back into the context's stream;
+ * default 0),
  * "zfold" (1: fused tiles of instances with a short padded z axis, Pz <= 8,
  * span all layers and do the z transform in registers, so sweeps skip the
  * separate z passes; default 0).

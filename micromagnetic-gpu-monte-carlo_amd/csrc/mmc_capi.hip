@@ -123,6 +123,10 @@ struct mmc_ctx {
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
     bool fuse_ok = false;             // every instance admits them
     bool zfold = false;               // fused tiles of short-z instances fold z in
+    bool concurrent = false;          // per-length launches of a pass on side streams
+    std::vector<hipStream_t> side;    // side streams (default priority) and the
+    hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
+    std::vector<hipEvent_t> join_ev;  // back into the context's stream
     // per instance (tyx, ncol, tyf): widest tiles that fit, and the tiles
     // chosen for the launch sizes (the autotuner's starting points)
     std::vector<std::array<int, 3>> tile_full, tile_default;
@@ -827,17 +831,63 @@ struct Bufs {
           Kh((const cpx<T> *)c->d_khat) {}
 };
 
+constexpr int kSideStreams = 3;   // with the context's own: four hardware queues
+
+int ensure_side_streams(mmc_ctx *c)
+{
+    if (!c->side.empty()) return MMC_OK;
+    HIPCHK(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    for (int k = 0; k < kSideStreams; ++k) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        c->side.push_back(s);
+        HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->join_ev.push_back(e);
+    }
+    return MMC_OK;
+}
+
+// Launch one kernel per segment of `l` (kernel variants serving disjoint
+// instances): in order on the context's stream, or with "concurrent" spread
+// over side streams forked from and joined back into it, so small
+// per-length launches of a batch overlap.  Captures into graphs as
+// parallel branches.
+template <typename F>
+int run_segments(mmc_ctx *c, const List &l, int kind, F &&launch)
+{
+    const size_t n = l.seg.size();
+    if (!c->concurrent || n <= 1 || !c->stream) {
+        for (const Segment &sg : l.seg)
+            HIPCHK(c, timed(c, kind, [&] { return launch(sg, c->stream); }, sg.bytes));
+        return MMC_OK;
+    }
+    int rc = ensure_side_streams(c);
+    if (rc) return rc;
+    const size_t ns = std::min(n - 1, c->side.size());
+    HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
+    for (size_t k = 0; k < ns; ++k) HIPCHK(c, hipStreamWaitEvent(c->side[k], c->fork_ev, 0));
+    for (size_t i = 0; i < n; ++i) {
+        hipStream_t s = i == 0 ? c->stream : c->side[(i - 1) % ns];
+        const Segment &sg = l.seg[i];
+        HIPCHK(c, timed(c, kind, [&] { return launch(sg, s); }, sg.bytes));
+    }
+    for (size_t k = 0; k < ns; ++k) {
+        HIPCHK(c, hipEventRecord(c->join_ev[k], c->side[k]));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev[k], 0));
+    }
+    return MMC_OK;
+}
+
 // m -> A (x pass of every instance with demag)
 template <typename T>
 int pass_xfwd(mmc_ctx *c)
 {
     const Bufs<T> b(c);
-    for (const Segment &sg : c->xl.seg)
-        HIPCHK(c, timed(c, K_XFWD, [&] {
-                   return launch_xfwd<T>(c->stream, c->dInst, c->xl.dev + sg.begin, sg.count, 0,
-                                         sg.fixed_n, sg.rows, b.m, b.A, b.tw, sg.lds);
-               }, sg.bytes));
-    return MMC_OK;
+    return run_segments(c, c->xl, K_XFWD, [&](const Segment &sg, hipStream_t st) {
+        return launch_xfwd<T>(st, c->dInst, c->xl.dev + sg.begin, sg.count, 0, sg.fixed_n, sg.rows,
+                              b.m, b.A, b.tw, sg.lds);
+    });
 }
 
 // z pass bytes over the instances list `l` serves (all with Pz > 1, or in
@@ -872,11 +922,11 @@ int pass_zyz(mmc_ctx *c, bool sweep = false)
     const List &l = sweep ? c->zls : c->zl;
     int rc = pass_zfwd<T>(c, sweep);
     if (rc) return rc;
-    for (const Segment &sg : c->yl.seg)
-        HIPCHK(c, timed(c, K_YMUL, [&] {
-                   return launch_ymul<T>(st, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
-                                         b.B, b.Kh, b.tw, sg.lds);
-               }, sg.bytes));
+    rc = run_segments(c, c->yl, K_YMUL, [&](const Segment &sg, hipStream_t s) {
+        return launch_ymul<T>(s, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, b.B,
+                              b.Kh, b.tw, sg.lds);
+    });
+    if (rc) return rc;
     if (l.n())
         HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, l.dev, l.n(), b.B, b.A, b.tw, l.lds); },
                         zbytes(c, sweep)));
@@ -894,12 +944,10 @@ int demag_chain(mmc_ctx *c)
     rc = pass_zyz<T>(c);
     if (rc) return rc;
     const Bufs<T> b(c);
-    for (const Segment &sg : c->xl.seg)
-        HIPCHK(c, timed(c, K_XINV, [&] {
-                   return launch_xinv<T>(c->stream, c->dInst, c->xl.dev + sg.begin, sg.count, sg.fixed_n,
-                                         sg.rows, b.A, b.h, b.tw, sg.lds);
-               }, sg.bytes));
-    return MMC_OK;
+    return run_segments(c, c->xl, K_XINV, [&](const Segment &sg, hipStream_t st) {
+        return launch_xinv<T>(st, c->dInst, c->xl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, b.A,
+                              b.h, b.tw, sg.lds);
+    });
 }
 
 bool fused_sweeps(const mmc_ctx *c) { return c->fuse && !c->metro_tiled; }
@@ -916,19 +964,17 @@ int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
         int rc = pass_zyz<T>(c, true);
         if (rc) return rc;
         const Bufs<T> b(c);
-        for (const Segment &sg : c->fl.seg)
-            HIPCHK(c, timed(c, K_XINVM, [&] {
-                       return launch_xinv_metro<T>(st, c->dInst, c->fl.dev + sg.begin, sg.count,
-                                                   sg.fixed_n, sg.rows, b.A, b.m, b.h, c->d_beta,
-                                                   c->d_ctl, sweep_off, b.tw, sg.lds);
-                   }, sg.bytes));
-        for (const Segment &sg : c->fl.seg)
-            HIPCHK(c, timed(c, K_MXFWD, [&] {
-                       return launch_metro_xfwd<T>(st, c->dInst, c->fl.dev + sg.begin, sg.count,
-                                                   sg.fixed_n, sg.rows, b.m, b.h, b.A, c->d_beta,
-                                                   c->d_ctl, sweep_off, b.tw, sg.lds);
-                   }, sg.bytes));
-        return MMC_OK;
+        rc = run_segments(c, c->fl, K_XINVM, [&](const Segment &sg, hipStream_t s) {
+            return launch_xinv_metro<T>(s, c->dInst, c->fl.dev + sg.begin, sg.count, sg.fixed_n,
+                                        sg.rows, b.A, b.m, b.h, c->d_beta, c->d_ctl, sweep_off, b.tw,
+                                        sg.lds);
+        });
+        if (rc) return rc;
+        return run_segments(c, c->fl, K_MXFWD, [&](const Segment &sg, hipStream_t s) {
+            return launch_metro_xfwd<T>(s, c->dInst, c->fl.dev + sg.begin, sg.count, sg.fixed_n,
+                                        sg.rows, b.m, b.h, b.A, c->d_beta, c->d_ctl, sweep_off, b.tw,
+                                        sg.lds);
+        });
     }
     int rc = demag_chain<T>(c);
     if (rc) return rc;
@@ -1198,6 +1244,7 @@ struct Schedule {
     bool fuse, tiled;
     int tiles;          // 0 chosen for the launch sizes, 1 minimal, 2 widest
     bool zfold;         // fused tiles of short-z instances fold z in
+    bool conc;          // per-length launches of a pass on side streams
 };
 
 // Fused tiles that fold z in where an instance admits them (zfold), else
@@ -1230,6 +1277,7 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
     }
     c->fuse = s.fuse;
     c->metro_tiled = s.tiled;
+    c->concurrent = s.conc;
     for (auto &in : c->inst) in.zf = 0;
     set_zfold(c, s.zfold);
     return retile(c, 0, 0);
@@ -1266,15 +1314,27 @@ template <typename T>
 int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
 {
     static const Schedule all[] = {
-        {"fused, launch-sized tiles", true, false, 0, false},
-        {"fused, minimal tiles", true, false, 1, false},
-        {"fused, widest tiles", true, false, 2, false},
-        {"fused, z folded", true, false, 0, true},
-        {"separate, launch-sized tiles", false, false, 0, false},
-        {"separate, minimal tiles", false, false, 1, false},
-        {"separate, widest tiles", false, false, 2, false},
-        {"tiled two-colour Metropolis", false, true, 0, false},
+        {"fused, launch-sized tiles", true, false, 0, false, false},
+        {"fused, minimal tiles", true, false, 1, false, false},
+        {"fused, widest tiles", true, false, 2, false, false},
+        {"fused, z folded", true, false, 0, true, false},
+        {"separate, launch-sized tiles", false, false, 0, false, false},
+        {"separate, minimal tiles", false, false, 1, false, false},
+        {"separate, widest tiles", false, false, 2, false, false},
+        {"tiled two-colour Metropolis", false, true, 0, false, false},
+        {"fused, launch-sized, concurrent", true, false, 0, false, true},
+        {"fused, minimal, concurrent", true, false, 1, false, true},
+        {"separate, launch-sized, concurrent", false, false, 0, false, true},
+        {"separate, minimal, concurrent", false, false, 1, false, true},
     };
+    // concurrent launches only matter where a pass has several kernel variants
+    bool multi = false;
+    {
+        std::map<int, int> px, py;
+        for (auto &in : c->inst)
+            if (in.cal_demag) { px[in.Px]++; py[in.Py]++; }
+        multi = px.size() > 1 || py.size() > 1;
+    }
     bool any_zf = false;
     {
         const bool z = c->zfold;
@@ -1298,7 +1358,8 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
     char line[160];
     for (int k = 0; k < (int)(sizeof(all) / sizeof(all[0])) && !rc; ++k) {
         const Schedule &s = all[k];
-        if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf))
+        if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
+            (s.conc && !multi))
             continue;
         rc = apply_schedule(c, s);
         double us = 0;
@@ -1351,6 +1412,9 @@ void mmc_destroy(mmc_ctx *c)
         if (p) (void)hipFree(p);
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls})
         if (l->dev) (void)hipFree(l->dev);
+    for (auto s : c->side) (void)hipStreamDestroy(s);
+    for (auto e : c->join_ev) (void)hipEventDestroy(e);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1762,6 +1826,12 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         for (auto &in : c->inst)
             if (in.tyf) in.tyf = (value > 1 || in.zf) ? fused_full(c, in) : 1;
         return retile(c, 0, 0);
+    }
+    if (k == "concurrent") {
+        c->concurrent = value != 0;
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+        return MMC_OK;
     }
     if (k == "zfold") {
         set_zfold(c, value != 0);
