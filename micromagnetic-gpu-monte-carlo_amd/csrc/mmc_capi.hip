@@ -28,9 +28,10 @@ constexpr double kPiTrunc = 3.14159265358;       // src/constant.h:21
 constexpr double kBoltzmann = 1.3806488E-23;     // src/constant.h:22
 
 const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy",
-                        "xinv_metro", "metro_xfwd"};
-constexpr int kNumKinds = 10;
-enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY, K_XINVM, K_MXFWD };
+                        "xinv_metro", "metro_xfwd", "zyz"};
+constexpr int kNumKinds = 11;
+enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY, K_XINVM, K_MXFWD,
+            K_ZYZ };
 
 struct Segment {
     int fixed_n = 0;   // common FFT length of the segment's kernel, 0 = generic
@@ -118,12 +119,14 @@ struct mmc_ctx {
     List fl;                          // fused x pass + Metropolis tiles
     List zls;                         // z passes of fused sweeps (instances
                                       // whose fused tiles do not fold z in)
+    List yzl;                         // z/y/Khat passes of short-z instances
     bool metro_tiled = false;         // both colours per launch (k_metro_tile)
     bool metro_tiled_ok = false;      // every instance admits the tiled kernel
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
     bool fuse_ok = false;             // every instance admits them
     bool zfold = false;               // fused tiles of short-z instances fold z in
     bool concurrent = false;          // per-length launches of a pass on side streams
+    bool zyz = true;                  // short-z instances take the k_zyz_n pass
     std::vector<hipStream_t> side;    // side streams (default priority) and the
     hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
     std::vector<hipEvent_t> join_ev;  // back into the context's stream
@@ -354,8 +357,9 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
     for (int c6 = 0; c6 < 6; ++c6) khat += in.kmode[c6] * (double)in.Hx * in.Pz * in.Py * b;
     switch (k) {
     case K_XFWD: case K_XINV: return in.cal_demag ? 3 * N * b + spec_x : 0.0;
-    case K_ZFWD: case K_ZINV: return (in.cal_demag && in.Pz > 1) ? spec_x + spec_z : 0.0;
-    case K_YMUL: return in.cal_demag ? 2 * spec_z + khat : 0.0;
+    case K_ZFWD: case K_ZINV: return (in.cal_demag && in.Pz > 1 && !in.zyz) ? spec_x + spec_z : 0.0;
+    case K_YMUL: return in.cal_demag && !in.zyz ? 2 * spec_z + khat : 0.0;
+    case K_ZYZ: return in.cal_demag && in.zyz ? 2 * spec_x + khat : 0.0;
     case K_METRO: {
         // one colour pass: all of m read (cell + neighbours), h of the
         // colour's cells and their updated m
@@ -451,9 +455,14 @@ void build_lists(mmc_ctx *c)
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
     std::map<std::pair<int, int>, Segment> fclass;   // (x length, row pairs) -> items
     std::map<std::pair<int, int>, std::vector<Work>> fitems;
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls}) reset(*l);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) reset(*l);
+    std::map<int, Segment> zyzclass;                 // y length -> items
+    std::map<int, std::vector<Work>> zyzitems;
     for (int i = 0; i < c->count; ++i) {
-        const Inst &in = c->inst[i];
+        Inst &inm = c->inst[i];
+        inm.zyz = c->zyz && !inm.zf && inm.cal_demag && inm.Pz > 1 && inm.Pz <= kZDirect &&
+                  fixed_fft_size(inm.Py) && (3 * (size_t)inm.Pz + 1) * inm.Py * cs <= c->lds_max;
+        const Inst &in = inm;
         const bool zdirect = in.Pz <= kZDirect;
         const int zstep = zdirect ? kThreads : in.tyz;
         const int zstep64 = zdirect ? kThreads : in.tyz64;
@@ -471,7 +480,13 @@ void build_lists(mmc_ctx *c)
             xs.lds = std::max(xs.lds, xlds);
             xs.bytes += inst_bytes(c, in, K_XFWD);
             c->xl.lds = std::max(c->xl.lds, xlds);
-            if (in.Pz > 1) {
+            if (in.zyz) {
+                for (int kx = 0; kx < in.Hx; ++kx) zyzitems[in.Py].push_back({i, kx, 0, 0});
+                Segment &zs = zyzclass[in.Py];
+                zs.fixed_n = in.Py;
+                zs.lds = std::max(zs.lds, (3 * (size_t)in.Pz + 1) * in.Py * cs);
+                zs.bytes += inst_bytes(c, in, K_ZYZ);
+            } else if (in.Pz > 1) {
                 for (int comp = 0; comp < 3; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
                         for (int y0 = 0; y0 < in.ny; y0 += zstep) {
@@ -482,17 +497,19 @@ void build_lists(mmc_ctx *c)
             }
             const int fn = fixed_fft_size(in.Py) ? in.Py : 0;
             const std::pair<int, int> yk(fn, fn ? in.ncol : 0);
-            for (int kz = 0; kz < in.Pz; ++kz)
+            for (int kz = 0; kz < in.Pz && !in.zyz; ++kz)
                 for (int kx0 = 0; kx0 < in.Hx; kx0 += in.ncol)
                     yitems[yk].push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
-            const size_t ylds = fn ? (3 * (size_t)in.ncol + 1) * in.Py * cs
-                                   : (2 * (size_t)3 * in.ncol + 1) * in.Py * cs;
-            Segment &sg = yclass[yk];
-            sg.fixed_n = fn;
-            sg.rows = yk.second;
-            sg.lds = std::max(sg.lds, ylds);
-            sg.bytes += inst_bytes(c, in, K_YMUL);
-            c->yl.lds = std::max(c->yl.lds, ylds);
+            if (!in.zyz) {
+                const size_t ylds = fn ? (3 * (size_t)in.ncol + 1) * in.Py * cs
+                                       : (2 * (size_t)3 * in.ncol + 1) * in.Py * cs;
+                Segment &sg = yclass[yk];
+                sg.fixed_n = fn;
+                sg.rows = yk.second;
+                sg.lds = std::max(sg.lds, ylds);
+                sg.bytes += inst_bytes(c, in, K_YMUL);
+                c->yl.lds = std::max(c->yl.lds, ylds);
+            }
             // tensor passes (FP64)
             for (int comp = 0; comp < 6; ++comp)
                 for (int z = 0; z < in.Pz; ++z)
@@ -534,7 +551,7 @@ void build_lists(mmc_ctx *c)
             fs.lds = std::max(fs.lds, (3 * (size_t)in.tyf + 1) * in.Px * cs);
             fs.bytes += inst_bytes(c, in, K_XINVM);
         }
-        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls})
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl})
             l->first.push_back(l->n());
     }
     c->zls.lds = c->zl.lds;
@@ -555,6 +572,13 @@ void build_lists(mmc_ctx *c)
         sg.count = (int)kv.second.size();
         c->xl.host.insert(c->xl.host.end(), kv.second.begin(), kv.second.end());
         c->xl.seg.push_back(sg);
+    }
+    for (auto &kv : zyzitems) {
+        Segment sg = zyzclass[kv.first];
+        sg.begin = c->yzl.n();
+        sg.count = (int)kv.second.size();
+        c->yzl.host.insert(c->yzl.host.end(), kv.second.begin(), kv.second.end());
+        c->yzl.seg.push_back(sg);
     }
     for (auto &kv : yitems) {
         Segment sg = yclass[kv.first];
@@ -733,7 +757,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
     HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -808,7 +832,7 @@ int retile(mmc_ctx *c, int tyx, int ncol)
         }
     }
     build_lists(c);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -925,6 +949,11 @@ int pass_zyz(mmc_ctx *c, bool sweep = false)
     rc = run_segments(c, c->yl, K_YMUL, [&](const Segment &sg, hipStream_t s) {
         return launch_ymul<T>(s, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, b.B,
                               b.Kh, b.tw, sg.lds);
+    });
+    if (rc) return rc;
+    rc = run_segments(c, c->yzl, K_ZYZ, [&](const Segment &sg, hipStream_t s) {
+        return launch_zyz<T>(s, c->dInst, c->yzl.dev + sg.begin, sg.count, sg.fixed_n, b.A, b.Kh, b.tw,
+                             sg.lds);
     });
     if (rc) return rc;
     if (l.n())
@@ -1410,7 +1439,7 @@ void mmc_destroy(mmc_ctx *c)
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
         if (p) (void)hipFree(p);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls})
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl})
         if (l->dev) (void)hipFree(l->dev);
     for (auto s : c->side) (void)hipStreamDestroy(s);
     for (auto e : c->join_ev) (void)hipEventDestroy(e);
@@ -1825,6 +1854,10 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         if (value < 1) return fail(c, MMC_EINVAL, "fuse_rows must be >= 1");
         for (auto &in : c->inst)
             if (in.tyf) in.tyf = (value > 1 || in.zf) ? fused_full(c, in) : 1;
+        return retile(c, 0, 0);
+    }
+    if (k == "zyz") {
+        c->zyz = value != 0;
         return retile(c, 0, 0);
     }
     if (k == "concurrent") {

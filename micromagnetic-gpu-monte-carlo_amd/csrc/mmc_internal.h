@@ -50,6 +50,7 @@ struct Inst {
     int tyf;              // complex rows per component of a fused x-pass +
                           // Metropolis tile (0: none)
     int zf;               // fused tiles span all layers and transform z too
+    int zyz;              // z, y and the Khat product in one pass (k_zyz_n)
     int kmode[6];         // Khat storage of each component: 0 zero, 1 real
                           // (imaginary part below rounding), 2 complex
     long N, NP;
@@ -108,6 +109,12 @@ template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_n, int cols, cpx<T> *buf, const cpx<T> *khat,
                        const cpx<T> *tw, size_t lds);
+// Short-z instances (1 < Pz <= kZDirect, Py = fixed_n with a fixed plan):
+// z, y, Khat product, y and z inverse in one pass over A; items (inst, kx).
+template <typename T>
+hipError_t launch_zyz(hipStream_t st, const Inst *I, const Work *W, int n,
+                      int fixed_n, cpx<T> *A, const cpx<T> *khat,
+                      const cpx<T> *tw, size_t lds);
 template <typename TO>
 hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W,
                               int n, const cpx<double> *ka, cpx<TO> *khat,

@@ -9,6 +9,8 @@
 //   k_zfwd / k_zinv z transform (3-D grids only)
 //   k_ymul          y forward transform, symmetric 3x3 k-space product
 //                   (src/demag_cu.cu:68-116), y inverse, in one pass
+//   k_zyz_n         z forward, y forward, Khat product, y and z inverse of one
+//                   x frequency (short z axes), the xz spectrum only in LDS
 //   k_yfwd_tensor   y transform of the tensor, scaled by 1/(PxPyPz)
 //   k_metro         Metropolis update of one colour class (mcKernel2,
 //                   src/mc_sampling.h:27-41)
@@ -630,6 +632,71 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
 {
     const Work w = W[blockIdx.x];
     ymul_fixed<T, N, C>(I[w.inst], w, bufb, khatb, twb);
+}
+
+// z forward, y forward, symmetric 3x3 product with Khat, y inverse and z
+// inverse of one x frequency, for instances with a short padded z axis
+// (1 < Pz <= kZDirect) and Py = N: item (inst, kx, 0, 0).  Reads and writes
+// the x spectrum A; the xz spectrum exists only in LDS, rows (kz, comp) =
+// kz*3 + comp, each z plane transformed as k_ymul_n<T, N, 1> transforms it.
+// Same arithmetic as k_zfwd + k_ymul_n + k_zinv (direct z paths).
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_zyz_n(const Inst *__restrict__ I,
+                                                    const Work *__restrict__ W,
+                                                    cpx<T> *__restrict__ Ab,
+                                                    const cpx<T> *__restrict__ khatb,
+                                                    const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int kx = w.a;
+    const int Pz = in.Pz, nz = in.nz, Ly = in.ny, H = in.Hx;
+    cpx<T> *tw = s + 3 * Pz * N;
+    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twy + e];
+    const cpx<T> *twz = twb + in.twz;
+    cpx<T> *A = Ab + in.a_off;
+    const long astride = (long)H * nz * Ly;
+    for (int e = threadIdx.x; e < 3 * N; e += kThreads) {
+        const int c = e / N, y = e - c * N;
+        cpx<T> v[kZDirect], o[kZDirect];
+        #pragma unroll
+        for (int z = 0; z < kZDirect; ++z)
+            v[z] = (z < nz && y < Ly) ? A[c * astride + ((long)kx * nz + z) * Ly + y]
+                                      : cpx<T>{T(0), T(0)};
+        zdft_regs<T, false>(v, o, nz, Pz, Pz, twz);
+        #pragma unroll
+        for (int kz = 0; kz < kZDirect; ++kz)
+            if (kz < Pz) s[(kz * 3 + c) * N + y] = y < Ly ? o[kz] : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3>(s + kz * 3 * N, tw);
+    {
+        const T *kr = reinterpret_cast<const T *>(khatb);
+        const long kstride = (long)H * Pz * N;
+        for (int e = threadIdx.x; e < Pz * N; e += kThreads) {
+            const int kz = e / N, ky = e - kz * N;
+            const long ki = ((long)kx * Pz + kz) * N + ky;
+            cpx<T> k6[6];
+            #pragma unroll
+            for (int c6 = 0; c6 < 6; ++c6) k6[c6] = khat_at(kr, in, c6, kstride, ki);
+            cpx<T> *r = s + kz * 3 * N + ky;
+            kmul_conj(r, r + N, r + 2 * N, k6);
+        }
+    }
+    __syncthreads();
+    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3>(s + kz * 3 * N, tw);
+    for (int e = threadIdx.x; e < 3 * Ly; e += kThreads) {
+        const int c = e / Ly, y = e - c * Ly;
+        cpx<T> v[kZDirect], o[kZDirect];
+        #pragma unroll
+        for (int kz = 0; kz < kZDirect; ++kz)
+            v[kz] = kz < Pz ? cconj(s[(kz * 3 + c) * N + y]) : cpx<T>{T(0), T(0)};
+        zdft_regs<T, true>(v, o, Pz, nz, Pz, twz);
+        #pragma unroll
+        for (int z = 0; z < kZDirect; ++z)
+            if (z < nz) A[c * astride + ((long)kx * nz + z) * Ly + y] = o[z];
+    }
 }
 
 // y forward, symmetric 3x3 product with Khat, y inverse.  Item (inst, kx0,
@@ -1665,6 +1732,23 @@ hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int 
     return post();
 }
 
+template <typename T>
+hipError_t launch_zyz(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
+                      cpx<T> *A, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 g(n), b(kThreads);
+#define MMC_ZYZ(NN)                                                                   \
+    if (fixed_n == NN) {                                                              \
+        MMC_LAUNCH((k_zyz_n<T, NN>), g, b, lds, st, I, W, A, khat, tw);               \
+        return post();                                                                \
+    }
+    MMC_ZYZ(64) MMC_ZYZ(96) MMC_ZYZ(128) MMC_ZYZ(192) MMC_ZYZ(256) MMC_ZYZ(384) MMC_ZYZ(512)
+    MMC_ZYZ(768) MMC_ZYZ(1024) MMC_ZYZ(2048)
+#undef MMC_ZYZ
+    return hipErrorInvalidValue;
+}
+
 template <typename TO>
 hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W, int n,
                               const cpx<double> *ka, cpx<TO> *khat, const cpx<double> *tw,
@@ -1848,6 +1932,16 @@ static hipError_t set_lds_t(int bytes)
     one((const void *)k_ymul_n<T, 2048, fixed_cols(2048)>);
     one((const void *)k_yfwd_tensor<T>);
     one((const void *)k_metro_tile<T>);
+    one((const void *)k_zyz_n<T, 64>);
+    one((const void *)k_zyz_n<T, 96>);
+    one((const void *)k_zyz_n<T, 128>);
+    one((const void *)k_zyz_n<T, 192>);
+    one((const void *)k_zyz_n<T, 256>);
+    one((const void *)k_zyz_n<T, 384>);
+    one((const void *)k_zyz_n<T, 512>);
+    one((const void *)k_zyz_n<T, 768>);
+    one((const void *)k_zyz_n<T, 1024>);
+    one((const void *)k_zyz_n<T, 2048>);
     constexpr int F = fused_rows_base<T>();
 #define MMC_FL(TT, NN, RR) one((const void *)k_xinv_metro<TT, NN, RR>); one((const void *)k_metro_xfwd<TT, NN, RR>);
     MMC_FUSED(MMC_FL, T, F)
@@ -1871,6 +1965,8 @@ hipError_t set_lds_limit(size_t bytes)
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                        cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \
+    template hipError_t launch_zyz<T>(hipStream_t, const Inst *, const Work *, int, int,       \
+                                      cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);       \
     template hipError_t launch_yfwd_tensor<T>(hipStream_t, const Inst *, const Work *, int,   \
                                               const cpx<double> *, cpx<T> *,                  \
                                               const cpx<double> *, size_t);                   \
