@@ -225,9 +225,10 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
  * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
  * tile, 2048 FP32 / 1024 FP64 complex per component over the x length),
- * "zyz" (1, the default: instances with a short padded z axis, 1 < Pz <= 8,
- * and a fixed-plan y length do the z transforms, the y transforms and the
- * Khat product in one pass per x frequency; 0: separate z and y passes),
+ * "zyz" (1: instances with a short padded z axis, 1 < Pz <= 8, and a
+ * fixed-plan y length do the z transforms, the y transforms and the Khat
+ * product in one pass per x frequency; 0, the default: separate z and y
+ * passes),
  * "concurrent" (1: the per-length launches of a pass over a batch run on
  * side streams forked from and joined back into the context's stream;
  * default 0),

@@ -36,18 +36,18 @@ __host__ __device__ constexpr bool fixed_fft_size(int n)
            n == 512 || n == 768 || n == 1024 || n == 2048;
 }
 
-template <typename T, int N, int ROWS, int NS, int R>
+template <typename T, int N, int ROWS, int NS, int R, int NT>
 __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
     constexpr int NB = N / R;
     constexpr int TOTAL = NB * ROWS;
-    constexpr int PER = (TOTAL + kThreads - 1) / kThreads;
+    constexpr int PER = (TOTAL + NT - 1) / NT;
     constexpr int TWS = N / (NS * R);
     cpx<T> v[PER][R];
     #pragma unroll
     for (int b = 0; b < PER; ++b) {
-        const int t = threadIdx.x + b * kThreads;
-        if ((b + 1) * kThreads <= TOTAL || t < TOTAL) {
+        const int t = threadIdx.x + b * NT;
+        if ((b + 1) * NT <= TOTAL || t < TOTAL) {
             const int row = t / NB, j = t % NB;
             #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = s[row * N + j + r * NB];
@@ -56,8 +56,8 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
     __syncthreads();
     #pragma unroll
     for (int b = 0; b < PER; ++b) {
-        const int t = threadIdx.x + b * kThreads;
-        if ((b + 1) * kThreads <= TOTAL || t < TOTAL) {
+        const int t = threadIdx.x + b * NT;
+        if ((b + 1) * NT <= TOTAL || t < TOTAL) {
             const int row = t / NB, j = t % NB;
             const int k = j % NS;
             if (NS > 1) {
@@ -73,25 +73,27 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
     __syncthreads();
 }
 
-template <typename T, int N, int ROWS, int NS, typename RL> struct FixedStages;
-template <typename T, int N, int ROWS, int NS> struct FixedStages<T, N, ROWS, NS, Radices<>> {
+template <typename T, int N, int ROWS, int NS, int NT, typename RL> struct FixedStages;
+template <typename T, int N, int ROWS, int NS, int NT>
+struct FixedStages<T, N, ROWS, NS, NT, Radices<>> {
     static __device__ __forceinline__ void run(cpx<T> *, const cpx<T> *) {}
 };
-template <typename T, int N, int ROWS, int NS, int R, int... Rest>
-struct FixedStages<T, N, ROWS, NS, Radices<R, Rest...>> {
+template <typename T, int N, int ROWS, int NS, int NT, int R, int... Rest>
+struct FixedStages<T, N, ROWS, NS, NT, Radices<R, Rest...>> {
     static __device__ __forceinline__ void run(cpx<T> *s, const cpx<T> *tw)
     {
-        fixed_stage<T, N, ROWS, NS, R>(s, tw);
-        FixedStages<T, N, ROWS, NS * R, Radices<Rest...>>::run(s, tw);
+        fixed_stage<T, N, ROWS, NS, R, NT>(s, tw);
+        FixedStages<T, N, ROWS, NS * R, NT, Radices<Rest...>>::run(s, tw);
     }
 };
 
 // Forward transform of ROWS contiguous rows of length N held in s (row
-// stride N).  Caller synchronises after writing s; synchronised on return.
-template <typename T, int N, int ROWS>
+// stride N) by a workgroup of NT threads.  Caller synchronises after
+// writing s; synchronised on return.
+template <typename T, int N, int ROWS, int NT = kThreads>
 __device__ __forceinline__ void fft_fixed(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
-    FixedStages<T, N, ROWS, 1, typename FixedPlan<N>::type>::run(s, tw);
+    FixedStages<T, N, ROWS, 1, NT, typename FixedPlan<N>::type>::run(s, tw);
 }
 
 // Columns per y-pass workgroup of the fixed path: about 3072 complex per tile.

@@ -126,7 +126,7 @@ struct mmc_ctx {
     bool fuse_ok = false;             // every instance admits them
     bool zfold = false;               // fused tiles of short-z instances fold z in
     bool concurrent = false;          // per-length launches of a pass on side streams
-    bool zyz = true;                  // short-z instances take the k_zyz_n pass
+    bool zyz = false;                 // short-z instances take the k_zyz_n pass
     std::vector<hipStream_t> side;    // side streams (default priority) and the
     hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
     std::vector<hipEvent_t> join_ev;  // back into the context's stream
@@ -247,7 +247,9 @@ bool choose_tiles(const mmc_ctx *c, Inst &in, size_t pref, size_t maxb)
     if (xfixed_size(in.Px) && (size_t)(xrows_for(c->tsize) + 1) * in.Px * cs <= maxb)
         pairs = xrows_for(c->tsize);
     in.tyx = 2 * pairs;
-    in.tyz = in.Pz <= kZDirect ? 1 : fit_tw(tymax, 2 * (size_t)in.Pz * cs, in.Pz * cs, pref, maxb);
+    // z tiles: up to 64 (FP32) / 32 (FP64) y columns, 256-512 bytes of
+    // contiguous y per z row
+    in.tyz = in.Pz <= kZDirect ? 1 : fit_tw(4 * tymax, 2 * (size_t)in.Pz * cs, in.Pz * cs, pref, maxb);
     in.ncol = fixed_fft_size(in.Py) ? fixed_cols(in.Py)
                                     : fit_tw(4, 2 * 3 * (size_t)in.Py * cs, in.Py * cs, pref, maxb);
     if (fixed_fft_size(in.Py) && (3 * (size_t)fixed_cols(in.Py) + 1) * in.Py * cs > maxb) in.ncol = 0;
@@ -1274,6 +1276,7 @@ struct Schedule {
     int tiles;          // 0 chosen for the launch sizes, 1 minimal, 2 widest
     bool zfold;         // fused tiles of short-z instances fold z in
     bool conc;          // per-length launches of a pass on side streams
+    bool zyz;           // short-z instances take the k_zyz_n pass
 };
 
 // Fused tiles that fold z in where an instance admits them (zfold), else
@@ -1307,6 +1310,7 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
     c->fuse = s.fuse;
     c->metro_tiled = s.tiled;
     c->concurrent = s.conc;
+    c->zyz = s.zyz;
     for (auto &in : c->inst) in.zf = 0;
     set_zfold(c, s.zfold);
     return retile(c, 0, 0);
@@ -1343,19 +1347,25 @@ template <typename T>
 int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
 {
     static const Schedule all[] = {
-        {"fused, launch-sized tiles", true, false, 0, false, false},
-        {"fused, minimal tiles", true, false, 1, false, false},
-        {"fused, widest tiles", true, false, 2, false, false},
-        {"fused, z folded", true, false, 0, true, false},
-        {"separate, launch-sized tiles", false, false, 0, false, false},
-        {"separate, minimal tiles", false, false, 1, false, false},
-        {"separate, widest tiles", false, false, 2, false, false},
-        {"tiled two-colour Metropolis", false, true, 0, false, false},
-        {"fused, launch-sized, concurrent", true, false, 0, false, true},
-        {"fused, minimal, concurrent", true, false, 1, false, true},
-        {"separate, launch-sized, concurrent", false, false, 0, false, true},
-        {"separate, minimal, concurrent", false, false, 1, false, true},
+        {"fused, launch-sized tiles", true, false, 0, false, false, false},
+        {"fused, minimal tiles", true, false, 1, false, false, false},
+        {"fused, widest tiles", true, false, 2, false, false, false},
+        {"fused, z folded", true, false, 0, true, false, false},
+        {"separate, launch-sized tiles", false, false, 0, false, false, false},
+        {"separate, minimal tiles", false, false, 1, false, false, false},
+        {"separate, widest tiles", false, false, 2, false, false, false},
+        {"tiled two-colour Metropolis", false, true, 0, false, false, false},
+        {"fused, launch-sized, concurrent", true, false, 0, false, true, false},
+        {"fused, minimal, concurrent", true, false, 1, false, true, false},
+        {"separate, launch-sized, concurrent", false, false, 0, false, true, false},
+        {"separate, minimal, concurrent", false, false, 1, false, true, false},
+        {"fused, minimal, one z/y pass", true, false, 1, false, false, true},
+        {"separate, minimal, one z/y pass", false, false, 1, false, false, true},
     };
+    // one z/y pass only matters for instances with a short padded z axis
+    bool any_zyz = false;
+    for (auto &in : c->inst)
+        any_zyz = any_zyz || (in.cal_demag && in.Pz > 1 && in.Pz <= kZDirect && fixed_fft_size(in.Py));
     // concurrent launches only matter where a pass has several kernel variants
     bool multi = false;
     {
@@ -1388,7 +1398,7 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
     for (int k = 0; k < (int)(sizeof(all) / sizeof(all[0])) && !rc; ++k) {
         const Schedule &s = all[k];
         if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
-            (s.conc && !multi))
+            (s.conc && !multi) || (s.zyz && !any_zyz))
             continue;
         rc = apply_schedule(c, s);
         double us = 0;

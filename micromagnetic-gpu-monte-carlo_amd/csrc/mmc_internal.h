@@ -109,6 +109,9 @@ template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_n, int cols, cpx<T> *buf, const cpx<T> *khat,
                        const cpx<T> *tw, size_t lds);
+// Threads of a k_zyz_n workgroup: its items hold all z planes of one x
+// frequency (few large items), so each gets twice the usual threads.
+constexpr int kZyzThreads = 512;
 // Short-z instances (1 < Pz <= kZDirect, Py = fixed_n with a fixed plan):
 // z, y, Khat product, y and z inverse in one pass over A; items (inst, kx).
 template <typename T>

@@ -641,7 +641,7 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
 // kz*3 + comp, each z plane transformed as k_ymul_n<T, N, 1> transforms it.
 // Same arithmetic as k_zfwd + k_ymul_n + k_zinv (direct z paths).
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_zyz_n(const Inst *__restrict__ I,
+__global__ __launch_bounds__(kZyzThreads) void k_zyz_n(const Inst *__restrict__ I,
                                                     const Work *__restrict__ W,
                                                     cpx<T> *__restrict__ Ab,
                                                     const cpx<T> *__restrict__ khatb,
@@ -653,11 +653,11 @@ __global__ __launch_bounds__(kThreads) void k_zyz_n(const Inst *__restrict__ I,
     const int kx = w.a;
     const int Pz = in.Pz, nz = in.nz, Ly = in.ny, H = in.Hx;
     cpx<T> *tw = s + 3 * Pz * N;
-    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twy + e];
+    for (int e = threadIdx.x; e < N; e += kZyzThreads) tw[e] = twb[in.twy + e];
     const cpx<T> *twz = twb + in.twz;
     cpx<T> *A = Ab + in.a_off;
     const long astride = (long)H * nz * Ly;
-    for (int e = threadIdx.x; e < 3 * N; e += kThreads) {
+    for (int e = threadIdx.x; e < 3 * N; e += kZyzThreads) {
         const int c = e / N, y = e - c * N;
         cpx<T> v[kZDirect], o[kZDirect];
         #pragma unroll
@@ -670,11 +670,12 @@ __global__ __launch_bounds__(kThreads) void k_zyz_n(const Inst *__restrict__ I,
             if (kz < Pz) s[(kz * 3 + c) * N + y] = y < Ly ? o[kz] : cpx<T>{T(0), T(0)};
     }
     __syncthreads();
-    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3>(s + kz * 3 * N, tw);
+    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3, kZyzThreads>(s + kz * 3 * N, tw);
     {
         const T *kr = reinterpret_cast<const T *>(khatb);
         const long kstride = (long)H * Pz * N;
-        for (int e = threadIdx.x; e < Pz * N; e += kThreads) {
+        #pragma unroll 2
+        for (int e = threadIdx.x; e < Pz * N; e += kZyzThreads) {
             const int kz = e / N, ky = e - kz * N;
             const long ki = ((long)kx * Pz + kz) * N + ky;
             cpx<T> k6[6];
@@ -685,8 +686,8 @@ __global__ __launch_bounds__(kThreads) void k_zyz_n(const Inst *__restrict__ I,
         }
     }
     __syncthreads();
-    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3>(s + kz * 3 * N, tw);
-    for (int e = threadIdx.x; e < 3 * Ly; e += kThreads) {
+    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3, kZyzThreads>(s + kz * 3 * N, tw);
+    for (int e = threadIdx.x; e < 3 * Ly; e += kZyzThreads) {
         const int c = e / Ly, y = e - c * Ly;
         cpx<T> v[kZDirect], o[kZDirect];
         #pragma unroll
@@ -1737,10 +1738,10 @@ hipError_t launch_zyz(hipStream_t st, const Inst *I, const Work *W, int n, int f
                       cpx<T> *A, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 g(n), b(kThreads);
+    const dim3 g(n);
 #define MMC_ZYZ(NN)                                                                   \
     if (fixed_n == NN) {                                                              \
-        MMC_LAUNCH((k_zyz_n<T, NN>), g, b, lds, st, I, W, A, khat, tw);               \
+        MMC_LAUNCH((k_zyz_n<T, NN>), g, dim3(kZyzThreads), lds, st, I, W, A, khat, tw);               \
         return post();                                                                \
     }
     MMC_ZYZ(64) MMC_ZYZ(96) MMC_ZYZ(128) MMC_ZYZ(192) MMC_ZYZ(256) MMC_ZYZ(384) MMC_ZYZ(512)
