@@ -130,7 +130,10 @@ def run_gpu(mod, kws, precision, steps, warmup, T=1400.0, graph=True, ksteps=Non
         c.set_option("graph", 1)
         c.set_option("graph_sweeps", 20)
     schedule = None
-    if autotune:
+    if isinstance(autotune, str):
+        c.set_schedule(autotune)
+        schedule = autotune
+    elif autotune:
         rep = c.autotune(autotune)
         log(rep.rstrip())
         schedule = rep.strip().splitlines()[-1].replace("chosen: ", "")
@@ -270,6 +273,9 @@ def main():
     ap.add_argument("--autotune", type=int, default=20,
                     help="sweeps per candidate schedule for the library's autotuner "
                          "(0: keep the default schedule)")
+    ap.add_argument("--schedule", default=None,
+                    help="apply this named sweep schedule to the measured film instead of "
+                         "autotuning (profiling runs)")
     ap.add_argument("--ksteps", type=int, default=None,
                     help="event-timed eager sweeps after the timed region (default "
                          "min(steps, 400); 0 skips them, as hardware-counter runs need)")
@@ -289,7 +295,8 @@ def main():
     kw = film256() if a.workload == "film256" else film1024x2()
     log(f"[bench] {a.workload} FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
     r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps,
-                  default_stream=a.default_stream, autotune=a.autotune)
+                  default_stream=a.default_stream,
+                  autotune=a.schedule if a.schedule else a.autotune)
     dom, roof = roofline(r32["kernels"], a.workload)
     extra = {}
     if not a.no_extra:

@@ -176,6 +176,13 @@ int mmc_sweeps(mmc_ctx *ctx, uint32_t stage, uint32_t sweep0, uint32_t count);
 int mmc_autotune(mmc_ctx *ctx, int sweeps_per_trial, char *report,
                  size_t report_len);
 
+/* Apply one of the schedules mmc_autotune times, by the name its report
+ * prints (e.g. "fused, minimal tiles"), without timing anything: reuse a
+ * tuned choice or pin one for profiling.  MMC_EUNSUP if the schedule needs
+ * kernels these instances cannot take, MMC_EINVAL for an unknown name.  No
+ * reference counterpart. */
+int mmc_set_schedule(mmc_ctx *ctx, const char *name);
+
 /* Single-instance convenience: set T, run one sweep. */
 int mmc_sweep(mmc_ctx *ctx, double T, uint32_t stage, uint32_t sweep);
 

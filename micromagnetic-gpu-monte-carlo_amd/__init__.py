@@ -110,7 +110,7 @@ EXPORTS = (
     "mmc_get_tensor", "mmc_mirror_tensor", "mmc_set_m", "mmc_get_m",
     "mmc_init_random_m", "mmc_field", "mmc_energy_density", "mmc_trial_delta_e", "mmc_energy",
     "mmc_set_temperature",
-    "mmc_sweeps", "mmc_autotune", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
+    "mmc_sweeps", "mmc_autotune", "mmc_set_schedule", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
     "mmc_kernel_stats", "mmc_kernel_bytes",
     "mmc_ovf_read", "mmc_ovf_write", "mmc_ovf_mesh_equal", "mmc_ini_parse",
 )
@@ -170,6 +170,7 @@ def lib() -> C.CDLL:
     L.mmc_trial_delta_e.argtypes = [vp, C.c_int, C.c_uint32, C.c_uint32, dp]
     L.mmc_energy_density.argtypes = [vp, C.c_int, C.c_int, dp]
     L.mmc_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
+    L.mmc_set_schedule.argtypes = [vp, C.c_char_p]
     L.mmc_energy.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mmc_set_temperature.argtypes = [vp, C.c_int, C.c_double]
     L.mmc_sweeps.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32]
@@ -373,6 +374,10 @@ class Context:
         buf = C.create_string_buffer(4096)
         self._check(lib().mmc_autotune(self._ptr, sweeps_per_trial, buf, len(buf)))
         return buf.value.decode()
+
+    def set_schedule(self, name: str):
+        """Apply a schedule named as in the autotuner's report."""
+        self._check(lib().mmc_set_schedule(self._ptr, name.encode()))
 
     def sweep(self, T: float, stage: int, sweep: int):
         self._check(lib().mmc_sweep(self._ptr, T, stage, sweep))

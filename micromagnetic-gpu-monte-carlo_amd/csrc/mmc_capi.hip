@@ -1343,10 +1343,7 @@ int time_sweeps(mmc_ctx *c, int sweeps, double *us)
     return MMC_OK;
 }
 
-template <typename T>
-int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
-{
-    static const Schedule all[] = {
+const Schedule kSchedules[] = {
         {"fused, launch-sized tiles", true, false, 0, false, false, false},
         {"fused, minimal tiles", true, false, 1, false, false, false},
         {"fused, widest tiles", true, false, 2, false, false, false},
@@ -1361,7 +1358,14 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
         {"separate, minimal, concurrent", false, false, 1, false, true, false},
         {"fused, minimal, one z/y pass", true, false, 1, false, false, true},
         {"separate, minimal, one z/y pass", false, false, 1, false, false, true},
-    };
+};
+constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
+
+// Can schedule `s` run on this context, and does it differ from the others
+// here (concurrency needs several kernel variants per pass, z folding and
+// the one-pass z/y transform need short-z instances)?
+bool schedule_useful(mmc_ctx *c, const Schedule &s)
+{
     // one z/y pass only matters for instances with a short padded z axis
     bool any_zyz = false;
     for (auto &in : c->inst)
@@ -1385,6 +1389,14 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
         }
         c->zfold = z;
     }
+    return !((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
+             (s.conc && !multi) || (s.zyz && !any_zyz));
+}
+
+template <typename T>
+int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
+{
+    const Schedule *all = kSchedules;
     void *save = nullptr;
     const size_t mbytes = std::max<size_t>(1, c->m_elems) * c->tsize;
     HIPCHK(c, hipMalloc(&save, mbytes));
@@ -1395,11 +1407,9 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
     int best = -1;
     double best_us = 0;
     char line[160];
-    for (int k = 0; k < (int)(sizeof(all) / sizeof(all[0])) && !rc; ++k) {
+    for (int k = 0; k < kNumSchedules && !rc; ++k) {
         const Schedule &s = all[k];
-        if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
-            (s.conc && !multi) || (s.zyz && !any_zyz))
-            continue;
+        if (!schedule_useful(c, s)) continue;
         rc = apply_schedule(c, s);
         double us = 0;
         if (!rc) rc = time_sweeps<T>(c, sweeps, &us);
@@ -1750,6 +1760,20 @@ int mmc_autotune(mmc_ctx *c, int sweeps_per_trial, char *report, size_t report_l
         report[n] = 0;
     }
     return rc;
+}
+
+int mmc_set_schedule(mmc_ctx *c, const char *name)
+{
+    if (!c || !name) return MMC_EINVAL;
+    for (int k = 0; k < kNumSchedules; ++k)
+        if (!strcmp(name, kSchedules[k].name)) {
+            const Schedule &s = kSchedules[k];
+            if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok))
+                return fail(c, MMC_EUNSUP, "schedule '%s' does not apply to these instances", name);
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            return apply_schedule(c, s);
+        }
+    return fail(c, MMC_EINVAL, "unknown schedule '%s'", name);
 }
 
 int mmc_sweep(mmc_ctx *c, double T, uint32_t stage, uint32_t sweep)

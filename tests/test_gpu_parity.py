@@ -410,7 +410,13 @@ def test_autotune_keeps_the_chain(mmc, geo):
     rep = c.autotune(4)
     print(rep)
     assert "chosen:" in rep
-    c.sweeps(1, 0, 6)
+    c.sweeps(1, 0, 3)
+    # any reported schedule can be applied by name mid-run: same chain
+    names = [ln.rsplit(None, 2)[0] for ln in rep.splitlines() if ln.endswith("us/sweep")]
+    c.set_schedule(names[-1])
+    with pytest.raises(mmc.MMCError):
+        c.set_schedule("no such schedule")
+    c.sweeps(1, 3, 3)
     assert np.array_equal(c.m(), ref)
 
 
