@@ -566,6 +566,9 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
 {
     constexpr int ROWS = 3 * C;
     constexpr int Q = (C * N + kThreads - 1) / kThreads;
+    // up to two k points per thread are prefetched; more would cost the
+    // registers that keep several workgroups resident per SIMD
+    constexpr bool PF = Q <= 2;
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     cpx<T> *tw = s + ROWS * N;
     const int kx0 = w.a, kz = w.b, ncol = w.c;
@@ -574,16 +577,19 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     const T *kr = reinterpret_cast<const T *>(khatb);
     const long cstride = (long)H * Pz * Ly;
     const long kstride = (long)H * Pz * N;
-    cpx<T> k6[Q][6];
-    #pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    auto load_k6 = [&](int q, cpx<T> *k6) {
         const int e = threadIdx.x + q * kThreads;
         const int col = e / N, ky = e % N;
         const bool ok = e < C * N && col < ncol;
         const long ki = ((long)(kx0 + col) * Pz + kz) * N + ky;
         #pragma unroll
         for (int c6 = 0; c6 < 6; ++c6)
-            k6[q][c6] = ok ? khat_at(kr, in, c6, kstride, ki) : cpx<T>{T(0), T(0)};
+            k6[c6] = ok ? khat_at(kr, in, c6, kstride, ki) : cpx<T>{T(0), T(0)};
+    };
+    cpx<T> k6p[PF ? Q : 1][6];
+    if (PF) {
+        #pragma unroll
+        for (int q = 0; q < (PF ? Q : 1); ++q) load_k6(q, k6p[q]);
     }
     for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twy + e];
     for (int e = threadIdx.x; e < ROWS * N; e += kThreads) {
@@ -605,8 +611,15 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
         const int e = threadIdx.x + q * kThreads;
         if (e < C * N) {
             const int col = e / N, ky = e % N;
+            cpx<T> k6[6];
+            if (PF) {
+                #pragma unroll
+                for (int c6 = 0; c6 < 6; ++c6) k6[c6] = k6p[PF ? q : 0][c6];
+            } else {
+                load_k6(q, k6);
+            }
             kmul_conj(s + (0 * C + col) * N + ky, s + (1 * C + col) * N + ky,
-                      s + (2 * C + col) * N + ky, k6[q]);
+                      s + (2 * C + col) * N + ky, k6);
         }
     }
     __syncthreads();
