@@ -221,13 +221,107 @@ __global__ __launch_bounds__(kThreads) void k_tensor(const Inst *__restrict__ I,
 // ============================================================================
 extern __shared__ __align__(16) unsigned char g_smem[];
 
+// Elements e = threadIdx.x + q*NT (q < ceil(MAXC/NT), e < count) of a
+// global -> LDS copy: ld(e) reads element e into registers, st(e, v) writes
+// it.  A rolled loop waits for each load before its LDS write, so a thread
+// pays one global latency per element; here every load of a chunk is issued
+// before the chunk's first write, so the latencies overlap.  Chunks hold at
+// most 128 bytes of loaded values per thread.
+template <int MAXC, int NT, typename V, typename LD, typename ST>
+__device__ __forceinline__ void staged_copy(int count, LD ld, ST st)
+{
+    constexpr int Q = (MAXC + NT - 1) / NT;
+    constexpr int CHM = 128 / (int)sizeof(V) > 0 ? 128 / (int)sizeof(V) : 1;
+    constexpr int CH = Q < CHM ? Q : CHM;
+    #pragma unroll
+    for (int q0 = 0; q0 < Q; q0 += CH) {
+        V v[CH];
+        #pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int e = threadIdx.x + (q0 + q) * NT;
+            if (q0 + q < Q && e < count) v[q] = ld(e);
+        }
+        #pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int e = threadIdx.x + (q0 + q) * NT;
+            if (q0 + q < Q && e < count) st(e, v[q]);
+        }
+    }
+}
+
+// The same for a count known only at run time: chunks of CH loads per
+// thread (at most 128 bytes), one global latency per chunk.
+template <typename V, int NT = kThreads, typename LD, typename ST>
+__device__ __forceinline__ void staged_copy_dyn(int count, LD ld, ST st)
+{
+    constexpr int CH = 128 / (int)sizeof(V) > 0 ? 128 / (int)sizeof(V) : 1;
+    for (int b = 0; b < count; b += CH * NT) {
+        V v[CH];
+        #pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int e = b + threadIdx.x + q * NT;
+            if (e < count) v[q] = ld(e);
+        }
+        #pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int e = b + threadIdx.x + q * NT;
+            if (e < count) st(e, v[q]);
+        }
+    }
+}
+
+// A staged copy split in two, for a few elements per thread (Q): the loads
+// are issued by staged_load, the LDS writes done by staged_store, so loads
+// of other data can be issued in between.  Vector-memory loads complete in
+// issue order, so data needed first is loaded first.
+template <int Q, typename V> struct Staged {
+    V v[Q];
+};
+
+template <int MAXC, int NT, typename V, typename LD>
+__device__ __forceinline__ Staged<(MAXC + NT - 1) / NT, V> staged_load(int count, LD ld)
+{
+    constexpr int Q = (MAXC + NT - 1) / NT;
+    Staged<Q, V> r;
+    #pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if (e < count) r.v[q] = ld(e);
+    }
+    return r;
+}
+
+template <int MAXC, int NT, typename V, typename ST>
+__device__ __forceinline__ void staged_store(int count, const Staged<(MAXC + NT - 1) / NT, V> &r,
+                                             ST st)
+{
+    constexpr int Q = (MAXC + NT - 1) / NT;
+    #pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = threadIdx.x + q * NT;
+        if (e < count) st(e, r.v[q]);
+    }
+}
+
+// Twiddle table of a compile-time length N into LDS, loads batched.
+template <typename T, int N, int NT = kThreads>
+__device__ __forceinline__ void stage_twiddles(cpx<T> *dst, const cpx<T> *__restrict__ src)
+{
+    staged_copy<N, NT, cpx<T>>(
+        N, [&](int e) { return src[e]; }, [&](int e, cpx<T> v) { dst[e] = v; });
+}
+
+// Two values loaded together by one thread of a staged copy.
+template <typename V> struct Pair2 { V a, b; };
+
 // Copy a twiddle table into LDS (after the two ping-pong buffers); callers
 // synchronise once data and twiddles are both loaded.
 template <typename T>
 __device__ __forceinline__ const cpx<T> *lds_twiddles(cpx<T> *dst, const cpx<T> *__restrict__ src,
                                                       int n)
 {
-    for (int e = threadIdx.x; e < n; e += kThreads) dst[e] = src[e];
+    staged_copy_dyn<cpx<T>>(
+        n, [&](int e) { return src[e]; }, [&](int e, cpx<T> v) { dst[e] = v; });
     return dst;
 }
 
@@ -262,16 +356,19 @@ __global__ __launch_bounds__(kThreads) void k_xfwd(const Inst *__restrict__ I,
     const cpx<T> *tw = lds_twiddles(s1 + (ty >> 1) * P, twb + in.twx, P);
     const int nyt = min(ty, Ly - y0);
     const int npair = (nyt + 1) >> 1;
-    for (int e = threadIdx.x; e < npair * P; e += kThreads) {
-        const int q = e / P, x = e - q * P;
-        T re = T(0), im = T(0);
-        if (x < Lx) {
-            const int ya = y0 + 2 * q;
-            re = src[(long)ya * Lx + x];
-            if (2 * q + 1 < nyt) im = src[(long)(ya + 1) * Lx + x];
-        }
-        s[q * P + x] = {re, im};
-    }
+    staged_copy_dyn<cpx<T>>(
+        npair * P,
+        [&](int e) {
+            const int q = e / P, x = e - q * P;
+            T re = T(0), im = T(0);
+            if (x < Lx) {
+                const int ya = y0 + 2 * q;
+                re = src[(long)ya * Lx + x];
+                if (2 * q + 1 < nyt) im = src[(long)(ya + 1) * Lx + x];
+            }
+            return cpx<T>{re, im};
+        },
+        [&](int e, cpx<T> v) { s[e] = v; });
     __syncthreads();
     s = fft_rows<T>(s, s1, npair, 1, P, in.plx, tw);
     // split the packed spectrum: Xa = (Z_k + conj Z_-k)/2, Xb = -i(Z_k - conj Z_-k)/2
@@ -306,14 +403,28 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
     const int npair = (nyt + 1) >> 1;
     // Z = Xa + i Xb on k < H, Hermitian completion above; stored conjugated
     // so the forward transform computes the inverse.
-    for (int e = threadIdx.x; e < H * npair; e += kThreads) {
-        const int k = e / npair, q = e - k * npair;
-        const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
-        const cpx<T> a = p[0];
-        const cpx<T> b = (2 * q + 1 < nyt) ? p[1] : cpx<T>{T(0), T(0)};
-        s[q * P + k] = {a.x - b.y, -(a.y + b.x)};
-        if (k > 0 && P - k >= H) s[q * P + P - k] = {a.x + b.y, a.y - b.x};
-    }
+    staged_copy_dyn<Pair2<cpx<T>>>(
+        H * npair,
+        [&](int e) {
+            const int k = e / npair, q = e - k * npair;
+            const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+            Pair2<cpx<T>> v;
+            v.a = p[0];
+            T bx = T(0), by = T(0);
+            if (2 * q + 1 < nyt) {
+                const cpx<T> b = p[1];
+                bx = b.x;
+                by = b.y;
+            }
+            v.b = {bx, by};
+            return v;
+        },
+        [&](int e, Pair2<cpx<T>> v) {
+            const int k = e / npair, q = e - k * npair;
+            const cpx<T> a = v.a, b = v.b;
+            s[q * P + k] = {a.x - b.y, -(a.y + b.x)};
+            if (k > 0 && P - k >= H) s[q * P + P - k] = {a.x + b.y, a.y - b.x};
+        });
     __syncthreads();
     s = fft_rows<T>(s, s1, npair, 1, P, in.plx, tw);
     for (int e = threadIdx.x; e < nyt * Lx; e += kThreads) {
@@ -343,17 +454,20 @@ __global__ __launch_bounds__(kThreads) void k_xfwd_n(const Inst *__restrict__ I,
     const T *src = realb + in.m_off + (long)c * in.N + (long)z * Lx * Ly;
     cpx<T> *dst = specb + in.a_off + (long)c * H * Lz * Ly;
     const int nyt = min(2 * ROWS, Ly - y0);
-    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
-    for (int e = threadIdx.x; e < ROWS * N; e += kThreads) {
-        const int q = e / N, x = e % N;
-        T re = T(0), im = T(0);
-        if (x < Lx && 2 * q < nyt) {
-            const int ya = y0 + 2 * q;
-            re = src[(long)ya * Lx + x];
-            if (2 * q + 1 < nyt) im = src[(long)(ya + 1) * Lx + x];
-        }
-        s[e] = {re, im};
-    }
+    stage_twiddles<T, N>(tw, twb + in.twx);
+    staged_copy<ROWS * N, kThreads, cpx<T>>(
+        ROWS * N,
+        [&](int e) {
+            const int q = e / N, x = e % N;
+            T re = T(0), im = T(0);
+            if (x < Lx && 2 * q < nyt) {
+                const int ya = y0 + 2 * q;
+                re = src[(long)ya * Lx + x];
+                if (2 * q + 1 < nyt) im = src[(long)(ya + 1) * Lx + x];
+            }
+            return cpx<T>{re, im};
+        },
+        [&](int e, cpx<T> v) { s[e] = v; });
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
     const int npair = (nyt + 1) >> 1;
@@ -385,16 +499,30 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
     T *dst = realb + in.h_off + (long)c * in.N + (long)z * Lx * Ly;
     const int nyt = min(2 * ROWS, Ly - y0);
     const int npair = (nyt + 1) >> 1;
-    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
+    stage_twiddles<T, N>(tw, twb + in.twx);
     for (int e = threadIdx.x; e < (ROWS - npair) * N; e += kThreads) s[npair * N + e] = {T(0), T(0)};
-    for (int e = threadIdx.x; e < H * npair; e += kThreads) {
-        const int k = e / npair, q = e - k * npair;
-        const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
-        const cpx<T> a = p[0];
-        const cpx<T> b = (2 * q + 1 < nyt) ? p[1] : cpx<T>{T(0), T(0)};
-        s[q * N + k] = {a.x - b.y, -(a.y + b.x)};
-        if (k > 0 && N - k >= H) s[q * N + N - k] = {a.x + b.y, a.y - b.x};
-    }
+    staged_copy<H * ROWS, kThreads, Pair2<cpx<T>>>(
+        H * npair,
+        [&](int e) {
+            const int k = e / npair, q = e - k * npair;
+            const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+            Pair2<cpx<T>> v;
+            v.a = p[0];
+            T bx = T(0), by = T(0);
+            if (2 * q + 1 < nyt) {
+                const cpx<T> b = p[1];
+                bx = b.x;
+                by = b.y;
+            }
+            v.b = {bx, by};
+            return v;
+        },
+        [&](int e, Pair2<cpx<T>> v) {
+            const int k = e / npair, q = e - k * npair;
+            const cpx<T> a = v.a, b = v.b;
+            s[q * N + k] = {a.x - b.y, -(a.y + b.x)};
+            if (k > 0 && N - k >= H) s[q * N + N - k] = {a.x + b.y, a.y - b.x};
+        });
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
     for (int e = threadIdx.x; e < nyt * Lx; e += kThreads) {
@@ -480,10 +608,19 @@ __global__ __launch_bounds__(kThreads) void k_zfwd(const Inst *__restrict__ I,
     cpx<T> *s1 = s + Pz * ty;
     const cpx<T> *tw = lds_twiddles(s1 + Pz * ty, twb + in.twz, Pz);
     const int nyt = min(ty, Ly - y0);
-    for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
-        const int zz = e / nyt, yy = e - zz * nyt;
-        s[e] = zz < Lz ? src[(long)zz * Ly + y0 + yy] : cpx<T>{T(0), T(0)};
-    }
+    staged_copy_dyn<cpx<T>>(
+        Pz * nyt,
+        [&](int e) {
+            const int zz = e / nyt, yy = e - zz * nyt;
+            T vx = T(0), vy = T(0);
+            if (zz < Lz) {
+                const cpx<T> t = src[(long)zz * Ly + y0 + yy];
+                vx = t.x;
+                vy = t.y;
+            }
+            return cpx<T>{vx, vy};
+        },
+        [&](int e, cpx<T> v) { s[e] = v; });
     __syncthreads();
     s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, tw);
     for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
@@ -515,10 +652,13 @@ __global__ __launch_bounds__(kThreads) void k_zinv(const Inst *__restrict__ I,
     }
     const cpx<T> *tw = lds_twiddles(s1 + Pz * in.tyz, twb + in.twz, Pz);
     const int nyt = min(in.tyz, Ly - y0);
-    for (int e = threadIdx.x; e < Pz * nyt; e += kThreads) {
-        const int zz = e / nyt, yy = e - zz * nyt;
-        s[e] = cconj(src[(long)zz * Ly + y0 + yy]);
-    }
+    staged_copy_dyn<cpx<T>>(
+        Pz * nyt,
+        [&](int e) {
+            const int zz = e / nyt, yy = e - zz * nyt;
+            return src[(long)zz * Ly + y0 + yy];
+        },
+        [&](int e, cpx<T> v) { s[e] = cconj(v); });
     __syncthreads();
     s = fft_rows<T>(s, s1, nyt, nyt, 1, in.plz, tw);
     for (int e = threadIdx.x; e < Lz * nyt; e += kThreads) {
@@ -566,9 +706,11 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
 {
     constexpr int ROWS = 3 * C;
     constexpr int Q = (C * N + kThreads - 1) / kThreads;
-    // up to two k points per thread are prefetched; more would cost the
-    // registers that keep several workgroups resident per SIMD
-    constexpr bool PF = Q <= 2;
+    // the k points of a thread are prefetched across the forward transform
+    // when their six values fit in 24 VGPRs (two FP32 / one FP64 point); more
+    // would cost the registers that keep several workgroups resident per
+    // SIMD, so they are loaded together right after it
+    constexpr bool PF = Q * 6 * (int)sizeof(cpx<T>) <= 96;
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     cpx<T> *tw = s + ROWS * N;
     const int kx0 = w.a, kz = w.b, ncol = w.c;
@@ -591,21 +733,30 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
         #pragma unroll
         for (int q = 0; q < (PF ? Q : 1); ++q) load_k6(q, k6p[q]);
     }
-    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twy + e];
-    for (int e = threadIdx.x; e < ROWS * N; e += kThreads) {
-        const int r = e / N, y = e % N;
-        const int comp = r / C, col = r % C;
-        // component-wise select: a conditionally loaded struct is kept in scratch
-        T vx = T(0), vy = T(0);
-        if (col < ncol && y < Ly) {
-            const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
-            vx = t.x;
-            vy = t.y;
-        }
-        s[e] = {vx, vy};
-    }
+    stage_twiddles<T, N>(tw, twb + in.twy);
+    staged_copy<ROWS * N, kThreads, cpx<T>>(
+        ROWS * N,
+        [&](int e) {
+            const int r = e / N, y = e % N;
+            const int comp = r / C, col = r % C;
+            // component-wise select: a conditionally loaded struct is kept in scratch
+            T vx = T(0), vy = T(0);
+            if (col < ncol && y < Ly) {
+                const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+                vx = t.x;
+                vy = t.y;
+            }
+            return cpx<T>{vx, vy};
+        },
+        [&](int e, cpx<T> v) { s[e] = v; });
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
+    // without prefetch, all k points' values are loaded before the first product
+    cpx<T> k6l[PF ? 1 : Q][6];
+    if (!PF) {
+        #pragma unroll
+        for (int q = 0; q < (PF ? 1 : Q); ++q) load_k6(q, k6l[q]);
+    }
     #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e = threadIdx.x + q * kThreads;
@@ -616,7 +767,8 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
                 #pragma unroll
                 for (int c6 = 0; c6 < 6; ++c6) k6[c6] = k6p[PF ? q : 0][c6];
             } else {
-                load_k6(q, k6);
+                #pragma unroll
+                for (int c6 = 0; c6 < 6; ++c6) k6[c6] = k6l[PF ? 0 : q][c6];
             }
             kmul_conj(s + (0 * C + col) * N + ky, s + (1 * C + col) * N + ky,
                       s + (2 * C + col) * N + ky, k6);
@@ -666,7 +818,7 @@ __global__ __launch_bounds__(kZyzThreads) void k_zyz_n(const Inst *__restrict__ 
     const int kx = w.a;
     const int Pz = in.Pz, nz = in.nz, Ly = in.ny, H = in.Hx;
     cpx<T> *tw = s + 3 * Pz * N;
-    for (int e = threadIdx.x; e < N; e += kZyzThreads) tw[e] = twb[in.twy + e];
+    stage_twiddles<T, N, kZyzThreads>(tw, twb + in.twy);
     const cpx<T> *twz = twb + in.twz;
     cpx<T> *A = Ab + in.a_off;
     const long astride = (long)H * nz * Ly;
@@ -735,17 +887,20 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
     const long cstride = (long)H * Pz * Ly;    // component stride in B
     const long kstride = (long)H * Pz * Py;    // component stride in Khat
     const int rows = 3 * ncol;
-    for (int e = threadIdx.x; e < rows * Py; e += kThreads) {
-        const int r = e / Py, y = e - r * Py;
-        const int comp = r / ncol, col = r - comp * ncol;
-        T vx = T(0), vy = T(0);
-        if (y < Ly) {
-            const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
-            vx = t.x;
-            vy = t.y;
-        }
-        s[e] = {vx, vy};
-    }
+    staged_copy_dyn<cpx<T>>(
+        rows * Py,
+        [&](int e) {
+            const int r = e / Py, y = e - r * Py;
+            const int comp = r / ncol, col = r - comp * ncol;
+            T vx = T(0), vy = T(0);
+            if (y < Ly) {
+                const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+                vx = t.x;
+                vy = t.y;
+            }
+            return cpx<T>{vx, vy};
+        },
+        [&](int e, cpx<T> v) { s[e] = v; });
     __syncthreads();
     s = fft_rows<T>(s, s1, rows, 1, Py, in.ply, tw);
     s1 = (s == s1) ? reinterpret_cast<cpx<T> *>(g_smem) : s1;
@@ -873,15 +1028,31 @@ __device__ __forceinline__ T quad_k0(const Inst &in, V3<T> a)
            T(2) * (k1 * a.x * a.y + k2 * a.x * a.z + k4 * a.y * a.z);
 }
 
-// Trial move and energy change of one cell.  dE (units mu0 Ms^2 V) =
-//   -d.(ex + dmi + b + h - K0 m) - kap[(m'.u)^2 - (m.u)^2] - [m'K0m' - mK0m]/2
-// with d = m' - m; trial m' uniform on the sphere from Philox counter
-// (cell, sweep, stage, 0); rr returns the counter's four outputs.
+template <typename T> struct Move {
+    V3<T> mn;   // trial direction
+    T u;        // acceptance draw
+};
+
+// Trial direction (uniform on the sphere) and acceptance draw of a cell from
+// Philox counter (cell, sweep, stage, 0): a function of the counter alone,
+// so it can be drawn before the cell's inputs arrive.
 template <typename T>
-__device__ __forceinline__ T trial_de(const Inst &in, const V3<T> &mi, const V3<T> &ex,
-                                      const V3<T> &dmi, const V3<T> &hi, int use_demag,
-                                      uint32_t cell, uint32_t sweep, uint32_t stage, V3<T> &mn,
-                                      uint4 &rr)
+__device__ __forceinline__ Move<T> trial_move(const Inst &in, uint32_t cell, uint32_t sweep,
+                                              uint32_t stage)
+{
+    const uint4 rr = philox10(make_uint4(cell, sweep, stage, 0u), in.key[0], in.key[1]);
+    const T zt = T(1) - T(2) * u01<T>(rr.x);
+    const T st = sqrt_t(max(T(0), T(1) - zt * zt));
+    T sn, cs;
+    sincospi_t(T(2) * u01<T>(rr.y), &sn, &cs);
+    return {{st * cs, st * sn, zt}, u01<T>(rr.z)};
+}
+
+// Energy change of moving cell m_i to mn (see trial_de).
+template <typename T>
+__device__ __forceinline__ T move_de(const Inst &in, const V3<T> &mi, const V3<T> &ex,
+                                     const V3<T> &dmi, const V3<T> &hi, int use_demag,
+                                     const V3<T> &mn)
 {
     V3<T> g = {ex.x + dmi.x + T(in.b[0]), ex.y + dmi.y + T(in.b[1]), ex.z + dmi.z + T(in.b[2])};
     if (use_demag) {
@@ -891,12 +1062,6 @@ __device__ __forceinline__ T trial_de(const Inst &in, const V3<T> &mi, const V3<
         g.y += hi.y - (k1 * mi.x + k3 * mi.y + k4 * mi.z);
         g.z += hi.z - (k2 * mi.x + k4 * mi.y + k5 * mi.z);
     }
-    rr = philox10(make_uint4(cell, sweep, stage, 0u), in.key[0], in.key[1]);
-    const T zt = T(1) - T(2) * u01<T>(rr.x);
-    const T st = sqrt_t(max(T(0), T(1) - zt * zt));
-    T sn, cs;
-    sincospi_t(T(2) * u01<T>(rr.y), &sn, &cs);
-    mn = {st * cs, st * sn, zt};
     const V3<T> d = {mn.x - mi.x, mn.y - mi.y, mn.z - mi.z};
     T dE = -(d.x * g.x + d.y * g.y + d.z * g.z);
     const T ux = T(in.u[0]), uy = T(in.u[1]), uz = T(in.u[2]);
@@ -907,18 +1072,100 @@ __device__ __forceinline__ T trial_de(const Inst &in, const V3<T> &mi, const V3<
     return dE;
 }
 
-// Metropolis decision for one cell: accept the trial move if dE <= 0 or
-// u < exp(-dE * beta), beta = mu0 Ms^2 V / (kB T), u from the counter's third
-// output.
+// Trial move of one cell (counter (cell, sweep, stage, 0)) and its energy
+// change, in units of mu0 Ms^2 V:
+//   dE = -d.(ex + dmi + b + h - K0 m) - kap[(m'.u)^2 - (m.u)^2] - [m'K0m' - mK0m]/2
+// with d = m' - m.
+template <typename T>
+__device__ __forceinline__ T trial_de(const Inst &in, const V3<T> &mi, const V3<T> &ex,
+                                      const V3<T> &dmi, const V3<T> &hi, int use_demag,
+                                      uint32_t cell, uint32_t sweep, uint32_t stage, V3<T> &mn)
+{
+    mn = trial_move<T>(in, cell, sweep, stage).mn;
+    return move_de(in, mi, ex, dmi, hi, use_demag, mn);
+}
+
+// Metropolis decision for a drawn move: accept if dE <= 0 or
+// u < exp(-dE * beta), beta = mu0 Ms^2 V / (kB T).
+template <typename T>
+__device__ __forceinline__ bool metro_accept(const Inst &in, const V3<T> &mi, const V3<T> &ex,
+                                             const V3<T> &dmi, const V3<T> &hi, int use_demag,
+                                             const Move<T> &mv, T bt)
+{
+    const T dE = move_de(in, mi, ex, dmi, hi, use_demag, mv.mn);
+    return dE <= T(0) || mv.u < exp_t(-dE * bt);
+}
+
+// Metropolis decision for one cell, move drawn here; mn = trial direction.
 template <typename T>
 __device__ __forceinline__ bool metro_cell(const Inst &in, const V3<T> &mi, const V3<T> &ex,
                                            const V3<T> &dmi, const V3<T> &hi, int use_demag,
                                            uint32_t cell, uint32_t sweep, uint32_t stage, T bt,
                                            V3<T> &mn)
 {
-    uint4 rr;
-    const T dE = trial_de(in, mi, ex, dmi, hi, use_demag, cell, sweep, stage, mn, rr);
-    return dE <= T(0) || u01<T>(rr.z) < exp_t(-dE * bt);
+    const Move<T> mv = trial_move<T>(in, cell, sweep, stage);
+    mn = mv.mn;
+    return metro_accept(in, mi, ex, dmi, hi, use_demag, mv, bt);
+}
+
+// The six neighbour values of cell (x,y,z) as stencil() reads them (zero for
+// a missing bond), loaded but not yet summed.
+template <typename T> struct Nbrs {
+    V3<T> xp, xm, yp, ym, zp, zm;
+};
+
+template <typename T>
+__device__ __forceinline__ Nbrs<T> gather_nbrs(const Inst &in, const T *m, int x, int y, int z)
+{
+    const long N = in.N;
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const V3<T> zero = {T(0), T(0), T(0)};
+    Nbrs<T> b = {zero, zero, zero, zero, zero, zero};
+    const long row = (long)nx * (y + (long)ny * z);
+    int j;
+    if ((j = nbr(x, +1, nx, in.pbc[0])) >= 0) b.xp = load_m(m, N, row + j);
+    if ((j = nbr(x, -1, nx, in.pbc[0])) >= 0) b.xm = load_m(m, N, row + j);
+    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) b.yp = load_m(m, N, x + (long)nx * (j + (long)ny * z));
+    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) b.ym = load_m(m, N, x + (long)nx * (j + (long)ny * z));
+    if (nz > 1) {
+        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) b.zp = load_m(m, N, x + (long)nx * (y + (long)ny * j));
+        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) b.zm = load_m(m, N, x + (long)nx * (y + (long)ny * j));
+    }
+    return b;
+}
+
+// Metropolis update of checkerboard cell i from its gathered neighbours (of
+// the other colour, so no workgroup writes them during the pass) and a drawn
+// move: writes global m when accepted, returns the cell's new value.
+template <typename T>
+__device__ __forceinline__ V3<T> metro_apply(const Inst &in, T *m, long i, const V3<T> &mi,
+                                             const Nbrs<T> &nb, const V3<T> &hi,
+                                             const Move<T> &mv, T bt)
+{
+    V3<T> ex, dmi;
+    bond_sums(in, nb.xp, nb.xm, nb.yp, nb.ym, nb.zp, nb.zm, ex, dmi);
+    if (metro_accept(in, mi, ex, dmi, hi, 1, mv, bt)) {
+        const long N = in.N;
+        m[i] = mv.mn.x;
+        m[N + i] = mv.mn.y;
+        m[2 * N + i] = mv.mn.z;
+        return mv.mn;
+    }
+    return mi;
+}
+
+// Cells per thread whose inputs k_metro_xfwd fetches before its LDS phase:
+// all QC when they fit (two FP32 / one FP64 cell), else none.
+template <typename T, int QC>
+__host__ __device__ constexpr int prefetch_cells()
+{
+    return QC <= (sizeof(T) == 4 ? 2 : 1) ? QC : 0;
+}
+// Cells whose inputs are loaded together when not prefetched.
+template <typename T>
+__host__ __device__ constexpr int gather_chunk()
+{
+    return sizeof(T) == 4 ? 2 : 1;
 }
 
 // One colour class, one thread per cell of the class (mcKernel2,
@@ -1114,28 +1361,6 @@ __global__ __launch_bounds__(kThreads) void k_metro_tile(const Inst *__restrict_
     }
 }
 
-// Metropolis decision for checkerboard cell (x, y, z) = flat index i with
-// demag field hi: neighbours from global m (they are of the other colour, so
-// no workgroup writes them during this pass).  Writes the accepted state to
-// global m and returns the cell's state after the update.
-template <typename T>
-__device__ __forceinline__ V3<T> metro_update(const Inst &in, T *m, int x, int y, int z, long i,
-                                              const V3<T> &mi, const V3<T> &hi, uint32_t sweep,
-                                              uint32_t stage, T bt)
-{
-    V3<T> ex, dmi, mn;
-    T cnt;
-    stencil(in, m, x, y, z, ex, dmi, cnt);
-    if (metro_cell(in, mi, ex, dmi, hi, 1, (uint32_t)i, sweep, stage, bt, mn)) {
-        const long N = in.N;
-        m[i] = mn.x;
-        m[N + i] = mn.y;
-        m[2 * N + i] = mn.z;
-        return mn;
-    }
-    return mi;
-}
-
 // Inverse x pass of all three components of a tile of real rows fused with
 // the first checkerboard colour of those rows.  Item (inst, 0, z0, y0): RZ
 // complex rows per component, each two real rows packed.  An instance with
@@ -1169,7 +1394,44 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
     const int R = RZ / nzl;
     const int nyt = min(2 * R, ny - y0);
     const int npair = (nyt + 1) >> 1;
-    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
+    stage_twiddles<T, N>(tw, twb + in.twx);
+    // Colour-0 cells of this thread: e = threadIdx.x + q*kThreads < ncell
+    // (at most QC).  Their m values, neighbours and trial moves are fetched
+    // after the transform, CH cells at a time with the loads issued together
+    // (holding them across the transform costs more occupancy than it saves
+    // latency).  Nothing in this kernel writes a cell they read: colour-0
+    // cells only have colour-1 neighbours.
+    T *m = mb + in.m_off;
+    const long NN = in.N;
+    const int hw = (nx + 1) >> 1;
+    const int plane = nyt * hw;
+    const int ncell = nzl * plane;
+    const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
+    constexpr int QC = (RZ * N + kThreads - 1) / kThreads;
+    constexpr int PFC = 0;
+    constexpr int CH = QC < gather_chunk<T>() ? QC : gather_chunk<T>();
+    auto cell_at = [&](int e, int &zl, int &yy, int &j) {
+        zl = e / plane;
+        const int r = e - zl * plane;
+        yy = r / hw;
+        j = r - yy * hw;
+    };
+    V3<T> cmi[CH];
+    Nbrs<T> cnb[CH];
+    Move<T> cmv[CH];
+    auto fetch = [&](int q, int k) {
+        const int e = threadIdx.x + q * kThreads;
+        int zl, yy, j;
+        cell_at(e, zl, yy, j);
+        const int z = z0 + zl, y = y0 + yy;
+        const int xa = 2 * j + ((y + z) & 1);
+        if (e < ncell && xa < nx) {
+            const long i = (long)nx * (y + (long)ny * z) + xa;
+            cmi[k] = load_m(m, NN, i);
+            cnb[k] = gather_nbrs(in, m, xa, y, z);
+            cmv[k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
+        }
+    };
     if (npair < R) {
         const int zr = (R - npair) * N;
         for (int e = threadIdx.x; e < 3 * nzl * zr; e += kThreads) {
@@ -1182,21 +1444,32 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
     if (!in.zf) {
         const long cstride = (long)H * nz * ny;
         const cpx<T> *src = specb + in.a_off;
-        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
-            const int c = e / per, r = e - c * per;
-            const int k = r / npair, q = r - k * npair;
-            const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
-            const cpx<T> a = p[0];
-            T bx = T(0), by = T(0);
-            if (2 * q + 1 < nyt) {
-                const cpx<T> b = p[1];
-                bx = b.x;
-                by = b.y;
-            }
-            cpx<T> *row = s + (c * RZ + q) * N;
-            row[k] = {a.x - by, -(a.y + bx)};
-            if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
-        }
+        staged_copy<3 * H * RZ, kThreads, Pair2<cpx<T>>>(
+            3 * per,
+            [&](int e) {
+                const int c = e / per, r = e - c * per;
+                const int k = r / npair, q = r - k * npair;
+                const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
+                Pair2<cpx<T>> v;
+                v.a = p[0];
+                T bx = T(0), by = T(0);
+                if (2 * q + 1 < nyt) {
+                    const cpx<T> b = p[1];
+                    bx = b.x;
+                    by = b.y;
+                }
+                v.b = {bx, by};
+                return v;
+            },
+            [&](int e, Pair2<cpx<T>> v) {
+                const int c = e / per, r = e - c * per;
+                const int k = r / npair, q = r - k * npair;
+                const cpx<T> a = v.a;
+                const T bx = v.b.x, by = v.b.y;
+                cpx<T> *row = s + (c * RZ + q) * N;
+                row[k] = {a.x - by, -(a.y + bx)};
+                if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
+            });
     } else {
         const int Pz = in.Pz;
         const long cstride = (long)H * Pz * ny;
@@ -1234,32 +1507,35 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
         if (yy & 1) return {-s[o].y, -s[RZ * N + o].y, -s[2 * RZ * N + o].y};
         return {s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
     };
-    T *m = mb + in.m_off;
     T *hc = hb + in.h_off;
-    const long NN = in.N;
-    const int hw = (nx + 1) >> 1;
     const long cst = (long)hw * ny * nz;
-    const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
     const T bt = T(beta[w.inst]);
-    const int plane = nyt * hw;
-    for (int e = threadIdx.x; e < nzl * plane; e += kThreads) {
-        const int zl = e / plane, r = e - zl * plane;
-        const int yy = r / hw, j = r - yy * hw;
-        const int z = z0 + zl, y = y0 + yy;
-        const int p0 = (y + z) & 1;
-        const int xa = 2 * j + p0, xb = 2 * j + (p0 ^ 1);
-        const long rowi = (long)nx * (y + (long)ny * z);
-        if (xb < nx) {
-            const V3<T> h1 = hval(zl, yy, xb);
-            const long o = ((long)z * ny + y) * hw + j;
-            hc[o] = h1.x;
-            hc[cst + o] = h1.y;
-            hc[2 * cst + o] = h1.z;
+    #pragma unroll
+    for (int q0 = 0; q0 < QC; q0 += CH) {
+        if (!PFC) {
+            #pragma unroll
+            for (int k = 0; k < CH; ++k)
+                if (q0 + k < QC) fetch(q0 + k, k);
         }
-        if (xa < nx) {
-            const long i = rowi + xa;
-            const V3<T> mi = load_m(m, NN, i);
-            metro_update(in, m, xa, y, z, i, mi, hval(zl, yy, xa), sweep, stage, bt);
+        #pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int e = threadIdx.x + (q0 + k) * kThreads;
+            if (q0 + k >= QC || e >= ncell) break;
+            int zl, yy, j;
+            cell_at(e, zl, yy, j);
+            const int z = z0 + zl, y = y0 + yy;
+            const int p0 = (y + z) & 1;
+            const int xa = 2 * j + p0, xb = 2 * j + (p0 ^ 1);
+            const long rowi = (long)nx * (y + (long)ny * z);
+            if (xb < nx) {
+                const V3<T> h1 = hval(zl, yy, xb);
+                const long o = ((long)z * ny + y) * hw + j;
+                hc[o] = h1.x;
+                hc[cst + o] = h1.y;
+                hc[2 * cst + o] = h1.z;
+            }
+            if (xa < nx)
+                metro_apply(in, m, rowi + xa, cmi[k], cnb[k], hval(zl, yy, xa), cmv[k], bt);
         }
     }
 }
@@ -1294,8 +1570,45 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
     const int nyt = min(2 * R, ny - y0);
     const long NN = in.N;
     T *m = mb + in.m_off;
-    for (int e = threadIdx.x; e < N; e += kThreads) tw[e] = twb[in.twx + e];
-    for (int e = threadIdx.x; e < 3 * RZ * N; e += kThreads) {
+    stage_twiddles<T, N>(tw, twb + in.twx);
+    // Colour-1 cells of this thread: e = threadIdx.x + q*kThreads < ncell
+    // (at most QC).  Their compact h, neighbours (colour 0, not written by
+    // this kernel) and trial moves are fetched CH cells at a time, loads
+    // issued together; when they fit (PFC = QC) they are fetched together
+    // with the tile's rows.
+    const T *hc = hb + in.h_off;
+    const int hw = (nx + 1) >> 1;
+    const long cst = (long)hw * ny * nz;
+    const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
+    const int plane = nyt * hw;
+    const int ncell = nzl * plane;
+    constexpr int QC = (RZ * N + kThreads - 1) / kThreads;
+    constexpr int PFC = prefetch_cells<T, QC>();
+    constexpr int CH = PFC ? PFC : (QC < gather_chunk<T>() ? QC : gather_chunk<T>());
+    auto cell_at = [&](int e, int &zl, int &yy, int &j) {
+        zl = e / plane;
+        const int r = e - zl * plane;
+        yy = r / hw;
+        j = r - yy * hw;
+    };
+    V3<T> chi[CH];
+    Nbrs<T> cnb[CH];
+    Move<T> cmv[CH];
+    auto fetch = [&](int q, int k) {
+        const int e = threadIdx.x + q * kThreads;
+        int zl, yy, j;
+        cell_at(e, zl, yy, j);
+        const int z = z0 + zl, y = y0 + yy;
+        const int x = 2 * j + ((y + z + 1) & 1);
+        if (e < ncell && x < nx) {
+            const long oh = ((long)z * ny + y) * hw + j;
+            chi[k] = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
+            cnb[k] = gather_nbrs(in, m, x, y, z);
+            cmv[k] = trial_move<T>(in, (uint32_t)((long)nx * (y + (long)ny * z) + x), sweep,
+                                   stage);
+        }
+    };
+    auto row_ld = [&](int e) {
         const int c = e / (RZ * N), r = e - c * (RZ * N);
         const int zr = r / N, x = r - zr * N;
         const int zl = zr / R, q = zr - zl * R;
@@ -1305,38 +1618,51 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
             re = row[x];
             if (2 * q + 1 < nyt) im = row[(long)nx + x];
         }
-        s[e] = {re, im};
+        return cpx<T>{re, im};
+    };
+    auto row_st = [&](int e, cpx<T> v) { s[e] = v; };
+    if (PFC) {
+        const auto rows = staged_load<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld);
+        #pragma unroll
+        for (int q = 0; q < PFC; ++q) fetch(q, q);
+        staged_store<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, rows, row_st);
+    } else {
+        staged_copy<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld, row_st);
     }
     __syncthreads();
     {
-        const T *hc = hb + in.h_off;
-        const int hw = (nx + 1) >> 1;
-        const long cst = (long)hw * ny * nz;
-        const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
         const T bt = T(beta[w.inst]);
-        const int plane = nyt * hw;
-        for (int e = threadIdx.x; e < nzl * plane; e += kThreads) {
-            const int zl = e / plane, r = e - zl * plane;
-            const int yy = r / hw, j = r - yy * hw;
-            const int z = z0 + zl, y = y0 + yy;
-            const int x = 2 * j + ((y + z + 1) & 1);
-            if (x >= nx) continue;
-            const int o = (zl * R + (yy >> 1)) * N + x;
-            const bool odd = yy & 1;
-            const V3<T> mi = odd ? V3<T>{s[o].y, s[RZ * N + o].y, s[2 * RZ * N + o].y}
-                                 : V3<T>{s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
-            const long oh = ((long)z * ny + y) * hw + j;
-            const V3<T> hi = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
-            const long i = (long)nx * (y + (long)ny * z) + x;
-            const V3<T> mn = metro_update(in, m, x, y, z, i, mi, hi, sweep, stage, bt);
-            if (odd) {
-                s[o].y = mn.x;
-                s[RZ * N + o].y = mn.y;
-                s[2 * RZ * N + o].y = mn.z;
-            } else {
-                s[o].x = mn.x;
-                s[RZ * N + o].x = mn.y;
-                s[2 * RZ * N + o].x = mn.z;
+        #pragma unroll
+        for (int q0 = 0; q0 < QC; q0 += CH) {
+            if (!PFC) {
+                #pragma unroll
+                for (int k = 0; k < CH; ++k)
+                    if (q0 + k < QC) fetch(q0 + k, k);
+            }
+            #pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                const int e = threadIdx.x + (q0 + k) * kThreads;
+                if (q0 + k >= QC || e >= ncell) break;
+                int zl, yy, j;
+                cell_at(e, zl, yy, j);
+                const int z = z0 + zl, y = y0 + yy;
+                const int x = 2 * j + ((y + z + 1) & 1);
+                if (x >= nx) continue;
+                const int o = (zl * R + (yy >> 1)) * N + x;
+                const bool odd = yy & 1;
+                const V3<T> mi = odd ? V3<T>{s[o].y, s[RZ * N + o].y, s[2 * RZ * N + o].y}
+                                     : V3<T>{s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
+                const long i = (long)nx * (y + (long)ny * z) + x;
+                const V3<T> mn = metro_apply(in, m, i, mi, cnb[k], chi[k], cmv[k], bt);
+                if (odd) {
+                    s[o].y = mn.x;
+                    s[RZ * N + o].y = mn.y;
+                    s[2 * RZ * N + o].y = mn.z;
+                } else {
+                    s[o].x = mn.x;
+                    s[RZ * N + o].x = mn.y;
+                    s[2 * RZ * N + o].x = mn.z;
+                }
             }
         }
     }
@@ -1612,8 +1938,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_e(const Inst *__restrict__ I
     stencil(in, m, x, y, z, ex, dmi, cnt);
     V3<T> hi = {T(0), T(0), T(0)};
     if (in.cal_demag) hi = load_m(hb + in.h_off, N, i);
-    uint4 rr;
-    const T dE = trial_de(in, mi, ex, dmi, hi, in.cal_demag, (uint32_t)i, sweep, stage, mn, rr);
+    const T dE = trial_de(in, mi, ex, dmi, hi, in.cal_demag, (uint32_t)i, sweep, stage, mn);
     out[i] = in.e_unit * double(dE);
 }
 
