@@ -232,16 +232,15 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
  * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
  * tile, 2048 FP32 / 1024 FP64 complex per component over the x length),
- * "zyz" (1: instances with a short padded z axis, 1 < Pz <= 8, and a
- * fixed-plan y length do the z transforms, the y transforms and the Khat
- * product in one pass per x frequency; 0, the default: separate z and y
- * passes),
  * "concurrent" (1: the per-length launches of a pass over a batch run on
  * side streams forked from and joined back into the context's stream;
  * default 0),
  * "zfold" (1: fused tiles of instances with a short padded z axis, Pz <= 8,
- * span all layers and do the z transform in registers, so sweeps skip the
- * separate z passes; default 0).
+ * that still take z transforms span all layers and do the z transform in
+ * registers, so sweeps skip the separate z passes; default 0).
+ * Films of 2-4 layers (Pz <= 8) with a y length of 64, 96, 128, 192, 256,
+ * 384, 512, 768, 1024 or 2048 after padding take no z transforms at all:
+ * one y pass per x frequency sums the z convolution directly ("yzconv").
  * Tile and fusion options only regroup independent work: results are
  * bit-identical.
  * Unknown keys return MMC_EINVAL. */
@@ -249,7 +248,7 @@ int mmc_set_option(mmc_ctx *ctx, const char *key, long value);
 
 /* Launches, summed device time [ms] and summed algorithmic HBM bytes of
  * one kernel kind ("xfwd", "zfwd", "ymul", "zinv", "xinv", "metro",
- * "energy", "tensor", "xinv_metro", "metro_xfwd", "zyz") recorded while "timing"
+ * "energy", "tensor", "xinv_metro", "metro_xfwd", "yzconv") recorded while "timing"
  * was on; total_bytes /
  * total_ms is the kind's achieved bandwidth.  Resets the counters when
  * reset != 0.  Instrumentation only; no reference counterpart (the

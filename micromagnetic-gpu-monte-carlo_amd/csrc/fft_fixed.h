@@ -36,7 +36,21 @@ __host__ __device__ constexpr bool fixed_fft_size(int n)
            n == 512 || n == 768 || n == 1024 || n == 2048;
 }
 
-template <typename T, int N, int ROWS, int NS, int R, int NT>
+// Twiddle exp(-2 pi i t / N) from a quarter table tw[u] = exp(-2 pi i u / N),
+// u < N/4, rotated by -i per quarter (exact); stage indices stay below 3N/4.
+template <typename T, int N>
+__device__ __forceinline__ cpx<T> twiddle_q(const cpx<T> *__restrict__ tw, int t)
+{
+    constexpr int Q4 = N / 4;
+    const int q = t / Q4, u = t - q * Q4;
+    const cpx<T> w = tw[u];
+    const cpx<T> r1 = {w.y, -w.x};    // -i w
+    const cpx<T> r2 = {-w.x, -w.y};   // -w
+    return q == 0 ? w : (q == 1 ? r1 : r2);
+}
+
+// QTW: tw is the quarter table (twiddle_q) instead of the full one.
+template <typename T, int N, int ROWS, int NS, int R, int NT, bool QTW = false>
 __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
     constexpr int NB = N / R;
@@ -62,7 +76,8 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
             const int k = j % NS;
             if (NS > 1) {
                 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * TWS]);
+                for (int r = 1; r < R; ++r)
+                    v[b][r] = cmul(v[b][r], QTW ? twiddle_q<T, N>(tw, r * k * TWS) : tw[r * k * TWS]);
             }
             Dft<T, R>::run(v[b]);
             const int o = (j - k) * R + k;
@@ -73,27 +88,28 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
     __syncthreads();
 }
 
-template <typename T, int N, int ROWS, int NS, int NT, typename RL> struct FixedStages;
-template <typename T, int N, int ROWS, int NS, int NT>
-struct FixedStages<T, N, ROWS, NS, NT, Radices<>> {
+template <typename T, int N, int ROWS, int NS, int NT, bool QTW, typename RL> struct FixedStages;
+template <typename T, int N, int ROWS, int NS, int NT, bool QTW>
+struct FixedStages<T, N, ROWS, NS, NT, QTW, Radices<>> {
     static __device__ __forceinline__ void run(cpx<T> *, const cpx<T> *) {}
 };
-template <typename T, int N, int ROWS, int NS, int NT, int R, int... Rest>
-struct FixedStages<T, N, ROWS, NS, NT, Radices<R, Rest...>> {
+template <typename T, int N, int ROWS, int NS, int NT, bool QTW, int R, int... Rest>
+struct FixedStages<T, N, ROWS, NS, NT, QTW, Radices<R, Rest...>> {
     static __device__ __forceinline__ void run(cpx<T> *s, const cpx<T> *tw)
     {
-        fixed_stage<T, N, ROWS, NS, R, NT>(s, tw);
-        FixedStages<T, N, ROWS, NS * R, NT, Radices<Rest...>>::run(s, tw);
+        fixed_stage<T, N, ROWS, NS, R, NT, QTW>(s, tw);
+        FixedStages<T, N, ROWS, NS * R, NT, QTW, Radices<Rest...>>::run(s, tw);
     }
 };
 
 // Forward transform of ROWS contiguous rows of length N held in s (row
-// stride N) by a workgroup of NT threads.  Caller synchronises after
-// writing s; synchronised on return.
-template <typename T, int N, int ROWS, int NT = kThreads>
+// stride N) by a workgroup of NT threads, twiddles from the full table (or
+// the quarter table, QTW).  Caller synchronises after writing s;
+// synchronised on return.
+template <typename T, int N, int ROWS, int NT = kThreads, bool QTW = false>
 __device__ __forceinline__ void fft_fixed(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
-    FixedStages<T, N, ROWS, 1, NT, typename FixedPlan<N>::type>::run(s, tw);
+    FixedStages<T, N, ROWS, 1, NT, QTW, typename FixedPlan<N>::type>::run(s, tw);
 }
 
 // Columns per y-pass workgroup of the fixed path: about 3072 complex per tile.

@@ -28,10 +28,10 @@ constexpr double kPiTrunc = 3.14159265358;       // src/constant.h:21
 constexpr double kBoltzmann = 1.3806488E-23;     // src/constant.h:22
 
 const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy",
-                        "xinv_metro", "metro_xfwd", "zyz"};
+                        "xinv_metro", "metro_xfwd", "yzconv"};
 constexpr int kNumKinds = 11;
 enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY, K_XINVM, K_MXFWD,
-            K_ZYZ };
+            K_YZC };
 
 struct Segment {
     int fixed_n = 0;   // common FFT length of the segment's kernel, 0 = generic
@@ -126,7 +126,6 @@ struct mmc_ctx {
     bool fuse_ok = false;             // every instance admits them
     bool zfold = false;               // fused tiles of short-z instances fold z in
     bool concurrent = false;          // per-length launches of a pass on side streams
-    bool zyz = false;                 // short-z instances take the k_zyz_n pass
     std::vector<hipStream_t> side;    // side streams (default priority) and the
     hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
     std::vector<hipEvent_t> join_ev;  // back into the context's stream
@@ -316,6 +315,11 @@ int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
         if (in.pbc[a] && n >= 3 && (n & 1)) in.ncolors = 3;
     }
     in.cal_demag = p.cal_demag ? 1 : 0;
+    // films of a few layers: the y pass sums the z convolution directly (no
+    // z transforms of the magnetisation spectrum or of the tensor)
+    in.zd = in.cal_demag && in.nz > 1 && in.nz <= kZConvMax && in.Pz <= kZDirect &&
+            fixed_fft_size(in.Py) &&
+            (3 * (size_t)in.nz * in.Py + in.Py / 4) * 2 * c->tsize <= c->lds_max;
     for (int c6 = 0; c6 < 6; ++c6) in.kmode[c6] = 2;   // until the tensor is built
     choose_mtile(c, in);
     // physics in reduced units (fields per Ms, energies per mu0 Ms^2 V)
@@ -355,13 +359,16 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
     const double N = (double)in.N;
     const double spec_x = 3.0 * in.Hx * in.nz * in.ny * cb;   // A
     const double spec_z = 3.0 * in.Hx * in.Pz * in.ny * cb;   // B
-    double khat = 0;   // zero planes are not read, real planes are half size
-    for (int c6 = 0; c6 < 6; ++c6) khat += in.kmode[c6] * (double)in.Hx * in.Pz * in.Py * b;
+    // zero planes are not read, real planes are half size; the direct z
+    // convolution reads the slots of offsets -(nz-1)..nz-1 only
+    const int kslots = in.zd ? std::min(in.Pz, 2 * in.nz - 1) : in.Pz;
+    double khat = 0;
+    for (int c6 = 0; c6 < 6; ++c6) khat += in.kmode[c6] * (double)in.Hx * kslots * in.Py * b;
     switch (k) {
     case K_XFWD: case K_XINV: return in.cal_demag ? 3 * N * b + spec_x : 0.0;
-    case K_ZFWD: case K_ZINV: return (in.cal_demag && in.Pz > 1 && !in.zyz) ? spec_x + spec_z : 0.0;
-    case K_YMUL: return in.cal_demag && !in.zyz ? 2 * spec_z + khat : 0.0;
-    case K_ZYZ: return in.cal_demag && in.zyz ? 2 * spec_x + khat : 0.0;
+    case K_ZFWD: case K_ZINV: return (in.cal_demag && in.Pz > 1 && !in.zd) ? spec_x + spec_z : 0.0;
+    case K_YMUL: return in.cal_demag && !in.zd ? 2 * spec_z + khat : 0.0;
+    case K_YZC: return in.cal_demag && in.zd ? 2 * spec_x + khat : 0.0;
     case K_METRO: {
         // one colour pass: all of m read (cell + neighbours), h of the
         // colour's cells and their updated m
@@ -398,8 +405,8 @@ bool xfixed(const mmc_ctx *c, const Inst &in)
 // instance `in` (0 when the instance cannot take the fused passes: no demag,
 // three colours, an x length without a fixed plan, or LDS too small for one
 // row pair).  With the "zfold" option, sets in.zf when a tile can span every
-// layer and fold the z transform in (Pz <= kZDirect and the layers divide
-// the tile).
+// layer and fold the z transform in (Pz <= kZDirect, the layers divide the
+// tile, and the instance has z transforms: not zd).
 int fused_full(const mmc_ctx *c, Inst &in)
 {
     in.zf = 0;
@@ -408,7 +415,7 @@ int fused_full(const mmc_ctx *c, Inst &in)
     const int r = fused_rows(c->tsize == 4 ? fused_rows_base<float>() : fused_rows_base<double>(),
                              in.Px);
     const bool fits = (3 * (size_t)r + 1) * in.Px * cs <= c->lds_max;
-    if (c->zfold && fits && in.Pz > 1 && in.Pz <= kZDirect && r % in.nz == 0) {
+    if (c->zfold && fits && in.Pz > 1 && in.Pz <= kZDirect && !in.zd && r % in.nz == 0) {
         in.zf = 1;
         return r;
     }
@@ -458,13 +465,10 @@ void build_lists(mmc_ctx *c)
     std::map<std::pair<int, int>, Segment> fclass;   // (x length, row pairs) -> items
     std::map<std::pair<int, int>, std::vector<Work>> fitems;
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) reset(*l);
-    std::map<int, Segment> zyzclass;                 // y length -> items
-    std::map<int, std::vector<Work>> zyzitems;
+    std::map<std::pair<int, int>, Segment> zdclass;  // (y length, layers) -> items
+    std::map<std::pair<int, int>, std::vector<Work>> zditems;
     for (int i = 0; i < c->count; ++i) {
-        Inst &inm = c->inst[i];
-        inm.zyz = c->zyz && !inm.zf && inm.cal_demag && inm.Pz > 1 && inm.Pz <= kZDirect &&
-                  fixed_fft_size(inm.Py) && (3 * (size_t)inm.Pz + 1) * inm.Py * cs <= c->lds_max;
-        const Inst &in = inm;
+        const Inst &in = c->inst[i];
         const bool zdirect = in.Pz <= kZDirect;
         const int zstep = zdirect ? kThreads : in.tyz;
         const int zstep64 = zdirect ? kThreads : in.tyz64;
@@ -482,12 +486,14 @@ void build_lists(mmc_ctx *c)
             xs.lds = std::max(xs.lds, xlds);
             xs.bytes += inst_bytes(c, in, K_XFWD);
             c->xl.lds = std::max(c->xl.lds, xlds);
-            if (in.zyz) {
-                for (int kx = 0; kx < in.Hx; ++kx) zyzitems[in.Py].push_back({i, kx, 0, 0});
-                Segment &zs = zyzclass[in.Py];
+            if (in.zd) {
+                const std::pair<int, int> zk(in.Py, in.nz);
+                for (int kx = 0; kx < in.Hx; ++kx) zditems[zk].push_back({i, kx, 0, 0});
+                Segment &zs = zdclass[zk];
                 zs.fixed_n = in.Py;
-                zs.lds = std::max(zs.lds, (3 * (size_t)in.Pz + 1) * in.Py * cs);
-                zs.bytes += inst_bytes(c, in, K_ZYZ);
+                zs.rows = in.nz;
+                zs.lds = std::max(zs.lds, (3 * (size_t)in.nz * in.Py + in.Py / 4) * cs);
+                zs.bytes += inst_bytes(c, in, K_YZC);
             } else if (in.Pz > 1) {
                 for (int comp = 0; comp < 3; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
@@ -499,10 +505,10 @@ void build_lists(mmc_ctx *c)
             }
             const int fn = fixed_fft_size(in.Py) ? in.Py : 0;
             const std::pair<int, int> yk(fn, fn ? in.ncol : 0);
-            for (int kz = 0; kz < in.Pz && !in.zyz; ++kz)
+            for (int kz = 0; kz < in.Pz && !in.zd; ++kz)
                 for (int kx0 = 0; kx0 < in.Hx; kx0 += in.ncol)
                     yitems[yk].push_back({i, kx0, kz, std::min(in.ncol, in.Hx - kx0)});
-            if (!in.zyz) {
+            if (!in.zd) {
                 const size_t ylds = fn ? (3 * (size_t)in.ncol + 1) * in.Py * cs
                                        : (2 * (size_t)3 * in.ncol + 1) * in.Py * cs;
                 Segment &sg = yclass[yk];
@@ -517,7 +523,7 @@ void build_lists(mmc_ctx *c)
                 for (int z = 0; z < in.Pz; ++z)
                     for (int y0 = 0; y0 < in.Py; y0 += in.tyx64) c->txl.host.push_back({i, comp, z, y0});
             c->txl.lds = std::max(c->txl.lds, (2 * (size_t)(in.tyx64 / 2) + 1) * in.Px * 16);
-            if (in.Pz > 1) {
+            if (in.Pz > 1 && !in.zd) {   // zd: the tensor's z slices stay untransformed
                 for (int comp = 0; comp < 6; ++comp)
                     for (int kx = 0; kx < in.Hx; ++kx)
                         for (int y0 = 0; y0 < in.Py; y0 += zstep64) c->tzl.host.push_back({i, comp, kx, y0});
@@ -575,8 +581,8 @@ void build_lists(mmc_ctx *c)
         c->xl.host.insert(c->xl.host.end(), kv.second.begin(), kv.second.end());
         c->xl.seg.push_back(sg);
     }
-    for (auto &kv : zyzitems) {
-        Segment sg = zyzclass[kv.first];
+    for (auto &kv : zditems) {
+        Segment sg = zdclass[kv.first];
         sg.begin = c->yzl.n();
         sg.count = (int)kv.second.size();
         c->yzl.host.insert(c->yzl.host.end(), kv.second.begin(), kv.second.end());
@@ -937,9 +943,10 @@ int pass_zfwd(mmc_ctx *c, bool sweep)
     return MMC_OK;
 }
 
-// A -> (z) -> y, Khat product, y -> (z) -> A.  In fused sweeps (`sweep`)
-// instances whose fused tiles fold z in skip the z passes: their tiles
-// read and write B directly.
+// A -> (z) -> y, Khat product, y -> (z) -> A; zd instances (a few layers)
+// take one y pass over A that sums the z convolution directly.  In fused
+// sweeps (`sweep`) instances whose fused tiles fold z in skip the z passes:
+// their tiles read and write B directly.
 template <typename T>
 int pass_zyz(mmc_ctx *c, bool sweep = false)
 {
@@ -953,9 +960,9 @@ int pass_zyz(mmc_ctx *c, bool sweep = false)
                               b.Kh, b.tw, sg.lds);
     });
     if (rc) return rc;
-    rc = run_segments(c, c->yzl, K_ZYZ, [&](const Segment &sg, hipStream_t s) {
-        return launch_zyz<T>(s, c->dInst, c->yzl.dev + sg.begin, sg.count, sg.fixed_n, b.A, b.Kh, b.tw,
-                             sg.lds);
+    rc = run_segments(c, c->yzl, K_YZC, [&](const Segment &sg, hipStream_t s) {
+        return launch_yzconv<T>(s, c->dInst, c->yzl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
+                                b.A, b.Kh, b.tw, sg.lds);
     });
     if (rc) return rc;
     if (l.n())
@@ -1276,7 +1283,6 @@ struct Schedule {
     int tiles;          // 0 chosen for the launch sizes, 1 minimal, 2 widest
     bool zfold;         // fused tiles of short-z instances fold z in
     bool conc;          // per-length launches of a pass on side streams
-    bool zyz;           // short-z instances take the k_zyz_n pass
 };
 
 // Fused tiles that fold z in where an instance admits them (zfold), else
@@ -1310,7 +1316,6 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
     c->fuse = s.fuse;
     c->metro_tiled = s.tiled;
     c->concurrent = s.conc;
-    c->zyz = s.zyz;
     for (auto &in : c->inst) in.zf = 0;
     set_zfold(c, s.zfold);
     return retile(c, 0, 0);
@@ -1344,32 +1349,26 @@ int time_sweeps(mmc_ctx *c, int sweeps, double *us)
 }
 
 const Schedule kSchedules[] = {
-        {"fused, launch-sized tiles", true, false, 0, false, false, false},
-        {"fused, minimal tiles", true, false, 1, false, false, false},
-        {"fused, widest tiles", true, false, 2, false, false, false},
-        {"fused, z folded", true, false, 0, true, false, false},
-        {"separate, launch-sized tiles", false, false, 0, false, false, false},
-        {"separate, minimal tiles", false, false, 1, false, false, false},
-        {"separate, widest tiles", false, false, 2, false, false, false},
-        {"tiled two-colour Metropolis", false, true, 0, false, false, false},
-        {"fused, launch-sized, concurrent", true, false, 0, false, true, false},
-        {"fused, minimal, concurrent", true, false, 1, false, true, false},
-        {"separate, launch-sized, concurrent", false, false, 0, false, true, false},
-        {"separate, minimal, concurrent", false, false, 1, false, true, false},
-        {"fused, minimal, one z/y pass", true, false, 1, false, false, true},
-        {"separate, minimal, one z/y pass", false, false, 1, false, false, true},
+        {"fused, launch-sized tiles", true, false, 0, false, false},
+        {"fused, minimal tiles", true, false, 1, false, false},
+        {"fused, widest tiles", true, false, 2, false, false},
+        {"fused, z folded", true, false, 0, true, false},
+        {"separate, launch-sized tiles", false, false, 0, false, false},
+        {"separate, minimal tiles", false, false, 1, false, false},
+        {"separate, widest tiles", false, false, 2, false, false},
+        {"tiled two-colour Metropolis", false, true, 0, false, false},
+        {"fused, launch-sized, concurrent", true, false, 0, false, true},
+        {"fused, minimal, concurrent", true, false, 1, false, true},
+        {"separate, launch-sized, concurrent", false, false, 0, false, true},
+        {"separate, minimal, concurrent", false, false, 1, false, true},
 };
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
 // Can schedule `s` run on this context, and does it differ from the others
-// here (concurrency needs several kernel variants per pass, z folding and
-// the one-pass z/y transform need short-z instances)?
+// here (concurrency needs several kernel variants per pass, z folding needs
+// short-z instances with z transforms)?
 bool schedule_useful(mmc_ctx *c, const Schedule &s)
 {
-    // one z/y pass only matters for instances with a short padded z axis
-    bool any_zyz = false;
-    for (auto &in : c->inst)
-        any_zyz = any_zyz || (in.cal_demag && in.Pz > 1 && in.Pz <= kZDirect && fixed_fft_size(in.Py));
     // concurrent launches only matter where a pass has several kernel variants
     bool multi = false;
     {
@@ -1390,7 +1389,7 @@ bool schedule_useful(mmc_ctx *c, const Schedule &s)
         c->zfold = z;
     }
     return !((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
-             (s.conc && !multi) || (s.zyz && !any_zyz));
+             (s.conc && !multi));
 }
 
 template <typename T>
@@ -1888,10 +1887,6 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
         if (value < 1) return fail(c, MMC_EINVAL, "fuse_rows must be >= 1");
         for (auto &in : c->inst)
             if (in.tyf) in.tyf = (value > 1 || in.zf) ? fused_full(c, in) : 1;
-        return retile(c, 0, 0);
-    }
-    if (k == "zyz") {
-        c->zyz = value != 0;
         return retile(c, 0, 0);
     }
     if (k == "concurrent") {

@@ -9,7 +9,9 @@
 //   m, h        [3][nz][ny][nx]            T       magnetisation, demag/Ms
 //   A           [3][Hx][nz][ny]            cpx<T>  x-spectrum, y fastest
 //   B           [3][Hx][Pz][ny]            cpx<T>  xz-spectrum (A if Pz==1; same pool)
-//   Khat        [6][Hx][Pz][Py]            cpx<T>  tensor spectrum / (PxPyPz)
+//   Khat        [6][Hx][Pz][Py]            cpx<T>  tensor spectrum / (PxPyPz), or with
+//                                                  zd the x-y spectra of the Pz z slices
+//                                                  / (PxPy)
 //   K           [6][Pz][Py][Px]            double  real-space tensor
 // with Hx = Px/2 + 1 (half spectrum of the real x transform).
 #pragma once
@@ -50,7 +52,8 @@ struct Inst {
     int tyf;              // complex rows per component of a fused x-pass +
                           // Metropolis tile (0: none)
     int zf;               // fused tiles span all layers and transform z too
-    int zyz;              // z, y and the Khat product in one pass (k_zyz_n)
+    int zd;               // z convolution summed directly in the y pass
+                          // (k_yzconv_n); Khat holds x-y spectra of z slices
     int kmode[6];         // Khat storage of each component: 0 zero, 1 real
                           // (imaginary part below rounding), 2 complex
     long N, NP;
@@ -109,15 +112,20 @@ template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_n, int cols, cpx<T> *buf, const cpx<T> *khat,
                        const cpx<T> *tw, size_t lds);
-// Threads of a k_zyz_n workgroup: its items hold all z planes of one x
+// Threads of a k_yzconv_n workgroup: its items hold all layers of one x
 // frequency (few large items), so each gets twice the usual threads.
-constexpr int kZyzThreads = 512;
-// Short-z instances (1 < Pz <= kZDirect, Py = fixed_n with a fixed plan):
-// z, y, Khat product, y and z inverse in one pass over A; items (inst, kx).
+constexpr int kYzThreads = 512;
+// Layers up to which the y pass sums the z convolution directly.
+constexpr int kZConvMax = 4;
+// Instances with 1 < nz <= kZConvMax layers, Pz <= kZDirect and Py = fixed_n
+// (a fixed plan) whose tensor spectrum holds the x-y spectra of its z
+// slices (Inst::zd): y transform, Khat product with the z convolution
+// summed directly, y inverse, in one pass over A; items (inst, kx), one
+// launch per (fixed_n, nz).
 template <typename T>
-hipError_t launch_zyz(hipStream_t st, const Inst *I, const Work *W, int n,
-                      int fixed_n, cpx<T> *A, const cpx<T> *khat,
-                      const cpx<T> *tw, size_t lds);
+hipError_t launch_yzconv(hipStream_t st, const Inst *I, const Work *W, int n,
+                         int fixed_n, int nz, cpx<T> *A, const cpx<T> *khat,
+                         const cpx<T> *tw, size_t lds);
 template <typename TO>
 hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W,
                               int n, const cpx<double> *ka, cpx<TO> *khat,

@@ -9,8 +9,8 @@
 //   k_zfwd / k_zinv z transform (3-D grids only)
 //   k_ymul          y forward transform, symmetric 3x3 k-space product
 //                   (src/demag_cu.cu:68-116), y inverse, in one pass
-//   k_zyz_n         z forward, y forward, Khat product, y and z inverse of one
-//                   x frequency (short z axes), the xz spectrum only in LDS
+//   k_yzconv_n      y transform, Khat product summed directly over z, y
+//                   inverse of one x frequency (films of a few layers)
 //   k_yfwd_tensor   y transform of the tensor, scaled by 1/(PxPyPz)
 //   k_metro         Metropolis update of one colour class (mcKernel2,
 //                   src/mc_sampling.h:27-41)
@@ -799,69 +799,134 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
     ymul_fixed<T, N, C>(I[w.inst], w, bufb, khatb, twb);
 }
 
-// z forward, y forward, symmetric 3x3 product with Khat, y inverse and z
-// inverse of one x frequency, for instances with a short padded z axis
-// (1 < Pz <= kZDirect) and Py = N: item (inst, kx, 0, 0).  Reads and writes
-// the x spectrum A; the xz spectrum exists only in LDS, rows (kz, comp) =
-// kz*3 + comp, each z plane transformed as k_ymul_n<T, N, 1> transforms it.
-// Same arithmetic as k_zfwd + k_ymul_n + k_zinv (direct z paths).
-template <typename T, int N>
-__global__ __launch_bounds__(kZyzThreads) void k_zyz_n(const Inst *__restrict__ I,
-                                                    const Work *__restrict__ W,
-                                                    cpx<T> *__restrict__ Ab,
-                                                    const cpx<T> *__restrict__ khatb,
-                                                    const cpx<T> *__restrict__ twb)
+// y forward transform, k-space product with a direct z convolution, y
+// inverse of one x frequency of an instance with a few layers (nz = NZ <=
+// kZConvMax, Pz <= kZDirect, Py = N).  The tensor's z slices are transformed
+// along x and y only, slot t of Khat holding slice offset t (mod Pz), so
+//   H^(kx, ky, z) = sum_{z' < nz} Khat[(z - z') mod Pz](kx, ky) M^(kx, ky, z')
+// for the nz real layers: the cyclic convolution along z that a z transform
+// of length Pz would compute, summed directly, with no z transform and no
+// xz spectrum.  Item (inst, kx, 0, 0); reads and writes the x spectrum A;
+// LDS row (z, comp) = z*3 + comp.
+// Waves per SIMD the compiler keeps room for in k_yzconv_n: six (three
+// workgroups per CU, 80 VGPRs) for FP32 films of two layers up to 1024
+// points, whose LDS allows three; the LDS of larger variants allows fewer
+// anyway.
+template <typename T, int N, int NZ>
+constexpr int yzconv_waves() { return sizeof(T) == 4 && NZ == 2 && N <= 1024 ? 6 : 1; }
+
+template <typename T, int N, int NZ>
+__global__ __launch_bounds__(kYzThreads) __attribute__((amdgpu_waves_per_eu(yzconv_waves<T, N, NZ>())))
+void k_yzconv_n(const Inst *__restrict__ I,
+                                                       const Work *__restrict__ W,
+                                                       cpx<T> *__restrict__ Ab,
+                                                       const cpx<T> *__restrict__ khatb,
+                                                       const cpx<T> *__restrict__ twb)
 {
+    constexpr int NT = kYzThreads;
+    constexpr int ROWS = 3 * NZ;
+    constexpr int ND = 2 * NZ - 1;                 // z offsets -(NZ-1) .. NZ-1
+    constexpr int Q = (N + NT - 1) / NT;           // k points per thread
+    // The rows and a quarter twiddle table (50 KB for two FP32 layers of
+    // 1024) leave room for three workgroups per CU, so a 1024-point x axis
+    // (513 items) runs in one round; the k points' tensor values are
+    // prefetched across the forward transform only when that costs no
+    // occupancy (24 VGPRs).
+    constexpr bool PF = Q == 1 && ND * 6 * (int)sizeof(cpx<T>) <= 96;
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + ROWS * N;                     // tw[u] = exp(-2 pi i u / N), u < N/4
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
     const int kx = w.a;
-    const int Pz = in.Pz, nz = in.nz, Ly = in.ny, H = in.Hx;
-    cpx<T> *tw = s + 3 * Pz * N;
-    stage_twiddles<T, N, kZyzThreads>(tw, twb + in.twy);
-    const cpx<T> *twz = twb + in.twz;
+    const int Pz = in.Pz, Ly = in.ny, H = in.Hx;
     cpx<T> *A = Ab + in.a_off;
-    const long astride = (long)H * nz * Ly;
-    for (int e = threadIdx.x; e < 3 * N; e += kZyzThreads) {
-        const int c = e / N, y = e - c * N;
-        cpx<T> v[kZDirect], o[kZDirect];
+    const long astride = (long)H * NZ * Ly;        // component stride in A
+    const T *kr = reinterpret_cast<const T *>(khatb);
+    const long kstride = (long)H * Pz * N;         // component stride in Khat
+    auto load_k = [&](int q, cpx<T> (&k6)[ND][6]) {
+        const int ky = threadIdx.x + q * NT;
         #pragma unroll
-        for (int z = 0; z < kZDirect; ++z)
-            v[z] = (z < nz && y < Ly) ? A[c * astride + ((long)kx * nz + z) * Ly + y]
-                                      : cpx<T>{T(0), T(0)};
-        zdft_regs<T, false>(v, o, nz, Pz, Pz, twz);
-        #pragma unroll
-        for (int kz = 0; kz < kZDirect; ++kz)
-            if (kz < Pz) s[(kz * 3 + c) * N + y] = y < Ly ? o[kz] : cpx<T>{T(0), T(0)};
+        for (int d = 0; d < ND; ++d) {
+            const int slot = (d - (NZ - 1) + Pz) % Pz;
+            const long ki = ((long)kx * Pz + slot) * N + ky;
+            #pragma unroll
+            for (int c6 = 0; c6 < 6; ++c6)
+                k6[d][c6] = ky < N ? khat_at(kr, in, c6, kstride, ki) : cpx<T>{T(0), T(0)};
+        }
+    };
+    stage_twiddles<T, N / 4, NT>(tw, twb + in.twy);
+    cpx<T> kp[PF ? Q : 1][ND][6];
+    auto row_ld = [&](int e) {
+        const int r = e / N, y = e - r * N;
+        const int z = r / 3, c = r - 3 * z;
+        T vx = T(0), vy = T(0);
+        if (y < Ly) {
+            const cpx<T> t = A[c * astride + ((long)kx * NZ + z) * Ly + y];
+            vx = t.x;
+            vy = t.y;
+        }
+        return cpx<T>{vx, vy};
+    };
+    auto row_st = [&](int e, cpx<T> v) { s[e] = v; };
+    if (PF) {
+        const auto rows = staged_load<ROWS * N, NT, cpx<T>>(ROWS * N, row_ld);
+        load_k(0, kp[0]);
+        staged_store<ROWS * N, NT, cpx<T>>(ROWS * N, rows, row_st);
+    } else {
+        staged_copy<ROWS * N, NT, cpx<T>>(ROWS * N, row_ld, row_st);
     }
     __syncthreads();
-    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3, kZyzThreads>(s + kz * 3 * N, tw);
-    {
-        const T *kr = reinterpret_cast<const T *>(khatb);
-        const long kstride = (long)H * Pz * N;
-        #pragma unroll 2
-        for (int e = threadIdx.x; e < Pz * N; e += kZyzThreads) {
-            const int kz = e / N, ky = e - kz * N;
-            const long ki = ((long)kx * Pz + kz) * N + ky;
-            cpx<T> k6[6];
+    fft_fixed<T, N, ROWS, NT, true>(s, tw);
+    // one k point at a time (a rolled loop keeps the next point's tensor
+    // loads from being hoisted, which would cost the third workgroup)
+    #pragma unroll 1
+    for (int q = 0; q < Q; ++q) {
+        const int ky = threadIdx.x + q * NT;
+        if (ky >= N) break;
+        cpx<T> kq[ND][6];
+        if (PF) {
             #pragma unroll
-            for (int c6 = 0; c6 < 6; ++c6) k6[c6] = khat_at(kr, in, c6, kstride, ki);
-            cpx<T> *r = s + kz * 3 * N + ky;
-            kmul_conj(r, r + N, r + 2 * N, k6);
+            for (int d = 0; d < ND; ++d)
+                #pragma unroll
+                for (int c6 = 0; c6 < 6; ++c6) kq[d][c6] = kp[0][d][c6];
+        } else {
+            load_k(q, kq);
+        }
+        cpx<T> mv[NZ][3];
+        #pragma unroll
+        for (int z = 0; z < NZ; ++z)
+            #pragma unroll
+            for (int c = 0; c < 3; ++c) mv[z][c] = s[(z * 3 + c) * N + ky];
+        #pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            cpx<T> h[3];
+            #pragma unroll
+            for (int zp = 0; zp < NZ; ++zp) {
+                const cpx<T> *k6 = kq[z - zp + NZ - 1];
+                const cpx<T> mx = mv[zp][0], my = mv[zp][1], mz = mv[zp][2];
+                const cpx<T> hx = cadd(cadd(cmul(mx, k6[0]), cmul(my, k6[1])), cmul(mz, k6[2]));
+                const cpx<T> hy = cadd(cadd(cmul(mx, k6[1]), cmul(my, k6[3])), cmul(mz, k6[4]));
+                const cpx<T> hz = cadd(cadd(cmul(mx, k6[2]), cmul(my, k6[4])), cmul(mz, k6[5]));
+                if (zp == 0) {
+                    h[0] = hx;
+                    h[1] = hy;
+                    h[2] = hz;
+                } else {
+                    h[0] = cadd(h[0], hx);
+                    h[1] = cadd(h[1], hy);
+                    h[2] = cadd(h[2], hz);
+                }
+            }
+            #pragma unroll
+            for (int c = 0; c < 3; ++c) s[(z * 3 + c) * N + ky] = cconj(h[c]);
         }
     }
     __syncthreads();
-    for (int kz = 0; kz < Pz; ++kz) fft_fixed<T, N, 3, kZyzThreads>(s + kz * 3 * N, tw);
-    for (int e = threadIdx.x; e < 3 * Ly; e += kZyzThreads) {
-        const int c = e / Ly, y = e - c * Ly;
-        cpx<T> v[kZDirect], o[kZDirect];
-        #pragma unroll
-        for (int kz = 0; kz < kZDirect; ++kz)
-            v[kz] = kz < Pz ? cconj(s[(kz * 3 + c) * N + y]) : cpx<T>{T(0), T(0)};
-        zdft_regs<T, true>(v, o, Pz, nz, Pz, twz);
-        #pragma unroll
-        for (int z = 0; z < kZDirect; ++z)
-            if (z < nz) A[c * astride + ((long)kx * nz + z) * Ly + y] = o[z];
+    fft_fixed<T, N, ROWS, NT, true>(s, tw);
+    for (int e = threadIdx.x; e < ROWS * Ly; e += NT) {
+        const int r = e / Ly, y = e - r * Ly;
+        const int z = r / 3, c = r - 3 * z;
+        A[c * astride + ((long)kx * NZ + z) * Ly + y] = cconj(s[r * N + y]);
     }
 }
 
@@ -930,7 +995,8 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
     }
 }
 
-// Tensor y transform (FP64), item (inst, comp, kx, kz); writes Khat / NP.
+// Tensor y transform (FP64), item (inst, comp, kx, kz); writes Khat / NP
+// (Khat / (Px Py) for zd instances, whose slices skip the z transform).
 template <typename TO>
 __global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict__ I,
                                                           const Work *__restrict__ W,
@@ -950,7 +1016,8 @@ __global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict
     for (int e = threadIdx.x; e < Py; e += kThreads) s[e] = src[e];
     __syncthreads();
     s = fft_rows<double>(s, s + Py, 1, 1, Py, in.ply, tw);
-    const double scale = 1.0 / (double)in.NP;
+    // no z transform for zd instances: their z slices are convolved directly
+    const double scale = 1.0 / (in.zd ? (double)in.Px * (double)in.Py : (double)in.NP);
     for (int e = threadIdx.x; e < Py; e += kThreads)
         dst[e] = {TO(s[e].x * scale), TO(s[e].y * scale)};
 }
@@ -2072,19 +2139,21 @@ hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int 
 }
 
 template <typename T>
-hipError_t launch_zyz(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
-                      cpx<T> *A, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+hipError_t launch_yzconv(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n, int nz,
+                         cpx<T> *A, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
     const dim3 g(n);
-#define MMC_ZYZ(NN)                                                                   \
-    if (fixed_n == NN) {                                                              \
-        MMC_LAUNCH((k_zyz_n<T, NN>), g, dim3(kZyzThreads), lds, st, I, W, A, khat, tw);               \
-        return post();                                                                \
+#define MMC_YZC(NN, ZZ)                                                                        \
+    if (fixed_n == NN && nz == ZZ) {                                                           \
+        MMC_LAUNCH((k_yzconv_n<T, NN, ZZ>), g, dim3(kYzThreads), lds, st, I, W, A, khat, tw);  \
+        return post();                                                                         \
     }
-    MMC_ZYZ(64) MMC_ZYZ(96) MMC_ZYZ(128) MMC_ZYZ(192) MMC_ZYZ(256) MMC_ZYZ(384) MMC_ZYZ(512)
-    MMC_ZYZ(768) MMC_ZYZ(1024) MMC_ZYZ(2048)
-#undef MMC_ZYZ
+#define MMC_YZC_N(NN) MMC_YZC(NN, 2) MMC_YZC(NN, 3) MMC_YZC(NN, 4)
+    MMC_YZC_N(64) MMC_YZC_N(96) MMC_YZC_N(128) MMC_YZC_N(192) MMC_YZC_N(256) MMC_YZC_N(384)
+    MMC_YZC_N(512) MMC_YZC_N(768) MMC_YZC_N(1024) MMC_YZC_N(2048)
+#undef MMC_YZC_N
+#undef MMC_YZC
     return hipErrorInvalidValue;
 }
 
@@ -2271,16 +2340,11 @@ static hipError_t set_lds_t(int bytes)
     one((const void *)k_ymul_n<T, 2048, fixed_cols(2048)>);
     one((const void *)k_yfwd_tensor<T>);
     one((const void *)k_metro_tile<T>);
-    one((const void *)k_zyz_n<T, 64>);
-    one((const void *)k_zyz_n<T, 96>);
-    one((const void *)k_zyz_n<T, 128>);
-    one((const void *)k_zyz_n<T, 192>);
-    one((const void *)k_zyz_n<T, 256>);
-    one((const void *)k_zyz_n<T, 384>);
-    one((const void *)k_zyz_n<T, 512>);
-    one((const void *)k_zyz_n<T, 768>);
-    one((const void *)k_zyz_n<T, 1024>);
-    one((const void *)k_zyz_n<T, 2048>);
+#define MMC_YZL(NN) one((const void *)k_yzconv_n<T, NN, 2>); one((const void *)k_yzconv_n<T, NN, 3>); \
+    one((const void *)k_yzconv_n<T, NN, 4>);
+    MMC_YZL(64) MMC_YZL(96) MMC_YZL(128) MMC_YZL(192) MMC_YZL(256) MMC_YZL(384) MMC_YZL(512)
+    MMC_YZL(768) MMC_YZL(1024) MMC_YZL(2048)
+#undef MMC_YZL
     constexpr int F = fused_rows_base<T>();
 #define MMC_FL(TT, NN, RR) one((const void *)k_xinv_metro<TT, NN, RR>); one((const void *)k_metro_xfwd<TT, NN, RR>);
     MMC_FUSED(MMC_FL, T, F)
@@ -2304,8 +2368,8 @@ hipError_t set_lds_limit(size_t bytes)
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                        cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \
-    template hipError_t launch_zyz<T>(hipStream_t, const Inst *, const Work *, int, int,       \
-                                      cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);       \
+    template hipError_t launch_yzconv<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
+                                         cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);    \
     template hipError_t launch_yfwd_tensor<T>(hipStream_t, const Inst *, const Work *, int,   \
                                               const cpx<double> *, cpx<T> *,                  \
                                               const cpx<double> *, size_t);                   \

@@ -21,7 +21,7 @@ CASES = [
     ("box3d_open", dict(nx=12, ny=10, nz=4)),
     ("box3d_pbcz", dict(nx=10, ny=8, nz=3, pbc_z=2)),
     ("ragged_3d", dict(nx=7, ny=9, nz=5, pbc_y=1)),
-    ("two_layer_3d_zyz", dict(nx=32, ny=32, nz=2)),
+    ("two_layer_3d_zconv", dict(nx=32, ny=32, nz=2)),
 ]
 
 
@@ -103,9 +103,11 @@ def test_init_random_matches_oracle(orc, mmc):
                                  dict(nx=9, ny=8, nz=1, pbc_x=2),
                                  dict(nx=8, ny=6, nz=3, pbc_z=1),
                                  dict(nx=64, ny=9, nz=1),
-                                 dict(nx=64, ny=6, nz=2, pbc_y=1)],
+                                 dict(nx=64, ny=6, nz=2, pbc_y=1),
+                                 dict(nx=64, ny=32, nz=2),
+                                 dict(nx=16, ny=64, nz=3, pbc_y=1, pbc_z=1)],
                          ids=["open", "pbc", "odd_pbc_3colors", "3d", "fused_open",
-                              "fused_3d"])
+                              "fused_3d", "zconv_fused", "zconv_pbc"])
 def test_sweep_trajectory_fp64(orc, mmc, geo):
     """Same Philox stream, same colour order, frozen demag: identical
     accept/reject decisions, so m agrees to rounding after several sweeps."""
@@ -491,21 +493,31 @@ def test_concurrent_segments_match_serial(mmc, fuse, graph):
                                                                 pbc_z=2)],
                          ids=["open_pz3", "pbcz_pz2", "open_pz5", "pbc_pz4"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
-def test_zyz_pass_matches_separate(mmc, geo, precision):
-    """Short-z instances: z, y, Khat product, y and z inverse in one pass per
-    x frequency (the xz spectrum only in LDS) give the bit-identical chain,
-    fields and energies of separate z and y passes, under fused and separate
-    sweep schedules, eager and graph-replayed."""
+def test_few_layers_sum_z_directly(orc, mmc, geo, precision):
+    """Films of 2-4 layers with a fixed-plan y length take no z transforms:
+    one y pass per x frequency sums the z convolution over the x-y spectra
+    of the tensor's z slices.  Fields and energies match the oracle's
+    z-transform convolution (FP64 within 1e-12, FP32 bulk within 2e-6 of
+    the largest field), and fused, separate, eager and graph-replayed
+    sweeps give one chain."""
     kw = film(**geo)
+    o, c = pair(orc, mmc, precision=precision, **kw)
+    assert c.kernel_bytes("yzconv") > 0
+    assert c.kernel_bytes("zfwd") == 0.0 and c.kernel_bytes("ymul") == 0.0
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 23)
+    o.set_m(m)
+    c.set_m(m)
+    tol = 1e-12 if precision == 0 else 2e-6
+    for tname, bits in (("demag", 16), ("all", 31)):
+        err = rel_inf(c.field(bits), o.field(bits))
+        assert err <= tol, f"{tname}: rel err {err:.3e}"
+    Eo, Eg = o.energy(), c.energy()
+    scale = sum(abs(Eo[k]) for k in ("exch", "dmi", "anis", "zeeman", "demag"))
+    assert abs(Eg["demag"] - Eo["demag"]) <= (1e-12 if precision == 0 else 1e-6) * scale
     out = []
-    for zyz, fuse, graph in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)):
+    for fuse, graph in ((0, 0), (1, 0), (1, 1)):
         c = mmc.Context(mmc.make_params(**kw), precision)
-        c.set_option("zyz", zyz)
-        if not zyz:
-            assert c.kernel_bytes("zyz") == 0.0
-        else:
-            assert c.kernel_bytes("zyz") > 0 and c.kernel_bytes("zfwd") == 0.0
-        c.build_tensor()
+        c.set_tensor(o.tensor())
         c.init_random_m()
         c.set_temperature(650.0)
         try:
