@@ -232,6 +232,14 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
  * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
  * tile, 2048 FP32 / 1024 FP64 complex per component over the x length),
+ * "rows" (1, with "fuse": every instance being a single-layer checkerboard
+ * film with a fused-kernel x length, a sweep is the y pass and one launch
+ * of whole-sweep row-pair tiles, each inverting the x spectrum of its pair
+ * and both neighbouring pairs, updating colour 0 on four rows (the halo
+ * rows' updates recomputed as their own tiles compute them) and colour 1 on
+ * its pair, and transforming the pair forward; the state alternates between
+ * two device buffers, settled into the first at the end of every call;
+ * default 0),
  * "concurrent" (1: the per-length launches of a pass over a batch run on
  * side streams forked from and joined back into the context's stream;
  * default 0),
@@ -248,9 +256,9 @@ int mmc_set_option(mmc_ctx *ctx, const char *key, long value);
 
 /* Launches, summed device time [ms] and summed algorithmic HBM bytes of
  * one kernel kind ("xfwd", "zfwd", "ymul", "zinv", "xinv", "metro",
- * "energy", "tensor", "xinv_metro", "metro_xfwd", "yzconv") recorded while "timing"
- * was on; total_bytes /
- * total_ms is the kind's achieved bandwidth.  Resets the counters when
+ * "energy", "tensor", "xinv_metro", "metro_xfwd", "yzconv", "row_sweep") recorded
+ * while "timing" was on; total_bytes / total_ms is the kind's achieved
+ * bandwidth.  Resets the counters when
  * reset != 0.  Instrumentation only; no reference counterpart (the
  * reference prints a CPU-clock "stages/s", src/main.cu:795-827). */
 int mmc_kernel_stats(mmc_ctx *ctx, const char *kind, long *launches,

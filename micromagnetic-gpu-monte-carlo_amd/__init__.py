@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmmc.so")
 FP64, FP32 = 0, 1
 EXCH, DMI, ANIS, ZEEMAN, DEMAG, ALL = 1, 2, 4, 8, 16, 31
 KINDS = ("tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy", "xinv_metro",
-         "metro_xfwd", "yzconv")
+         "metro_xfwd", "yzconv", "row_sweep")
 
 
 class MMCError(RuntimeError):

@@ -28,10 +28,10 @@ constexpr double kPiTrunc = 3.14159265358;       // src/constant.h:21
 constexpr double kBoltzmann = 1.3806488E-23;     // src/constant.h:22
 
 const char *kKinds[] = {"tensor", "xfwd", "zfwd", "ymul", "zinv", "xinv", "metro", "energy",
-                        "xinv_metro", "metro_xfwd", "yzconv"};
-constexpr int kNumKinds = 11;
+                        "xinv_metro", "metro_xfwd", "yzconv", "row_sweep"};
+constexpr int kNumKinds = 12;
 enum Kind { K_TENSOR, K_XFWD, K_ZFWD, K_YMUL, K_ZINV, K_XINV, K_METRO, K_ENERGY, K_XINVM, K_MXFWD,
-            K_YZC };
+            K_YZC, K_ROWS };
 
 struct Segment {
     int fixed_n = 0;   // common FFT length of the segment's kernel, 0 = generic
@@ -119,7 +119,11 @@ struct mmc_ctx {
     List fl;                          // fused x pass + Metropolis tiles
     List zls;                         // z passes of fused sweeps (instances
                                       // whose fused tiles do not fold z in)
-    List yzl;                         // z/y/Khat passes of short-z instances
+    List yzl;                         // y passes with the direct z convolution
+    List rl;                          // whole-sweep row-pair tiles (k_row_sweep)
+    bool rows = false;                // sweeps run k_row_sweep after the y pass
+    bool rows_ok = false;             // every instance admits it
+    void *d_m2 = nullptr;             // second state buffer of row sweeps
     bool metro_tiled = false;         // both colours per launch (k_metro_tile)
     bool metro_tiled_ok = false;      // every instance admits the tiled kernel
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
@@ -381,6 +385,9 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
     // read (metro_xfwd)
     case K_XINVM: case K_MXFWD:
         return in.cal_demag ? (in.zf ? spec_z : spec_x) + 6 * N * b : 0.0;
+    // whole sweep of row pairs: spectrum read and written, m read (cells and
+    // neighbours) and written
+    case K_ROWS: return in.cal_demag ? 2 * spec_x + 6 * N * b : 0.0;
     case K_TENSOR: return 6.0 * in.NP * 8;
     }
     return 0.0;
@@ -423,6 +430,15 @@ int fused_full(const mmc_ctx *c, Inst &in)
     return (4 * (size_t)in.Px * cs <= c->lds_max) ? 1 : 0;
 }
 
+// Can instance `in` take whole-sweep row-pair tiles (k_row_sweep)?  A
+// single-layer checkerboard film with demag, a fused-kernel x length and
+// room for three row pairs and the twiddles in LDS.
+bool row_ok(const mmc_ctx *c, const Inst &in)
+{
+    return in.cal_demag && in.nz == 1 && in.ncolors == 2 && xfixed_size(in.Px) &&
+           10 * (size_t)in.Px * 2 * c->tsize <= c->lds_max;
+}
+
 // Workgroups are dispatched to the 8 XCDs round-robin by index, and each
 // XCD has its own L2.  x-pass items that touch the same 128-byte lines of a
 // y-contiguous spectrum (consecutive row tiles: `share` items per line) are
@@ -462,9 +478,13 @@ void build_lists(mmc_ctx *c)
     std::map<std::pair<int, int>, std::vector<Work>> yitems;
     std::map<std::pair<int, int>, Segment> xclass;   // (fixed length, rows) -> items
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
+    std::map<int, Segment> rclass;                   // x length -> row-sweep items
+    std::map<int, std::vector<Work>> ritems;
+    c->rows_ok = !c->inst.empty();
+    for (auto &in : c->inst) c->rows_ok = c->rows_ok && row_ok(c, in);
     std::map<std::pair<int, int>, Segment> fclass;   // (x length, row pairs) -> items
     std::map<std::pair<int, int>, std::vector<Work>> fitems;
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) reset(*l);
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl, &c->rl}) reset(*l);
     std::map<std::pair<int, int>, Segment> zdclass;  // (y length, layers) -> items
     std::map<std::pair<int, int>, std::vector<Work>> zditems;
     for (int i = 0; i < c->count; ++i) {
@@ -544,6 +564,13 @@ void build_lists(mmc_ctx *c)
                 for (int x0 = 0; x0 < in.nx; x0 += in.mtx) c->mtl.host.push_back({i, x0, y0, 0});
             c->mtl.lds = std::max(c->mtl.lds, mtile_lds(c, in.mtx, in.mty, in.nz));
         }
+        if (c->rows_ok) {
+            for (int y0 = 0; y0 < in.ny; y0 += 2) ritems[in.Px].push_back({i, 0, 0, y0});
+            Segment &rs = rclass[in.Px];
+            rs.fixed_n = in.Px;
+            rs.lds = std::max(rs.lds, 10 * (size_t)in.Px * cs);
+            rs.bytes += inst_bytes(c, in, K_ROWS);
+        }
         if (in.tyf) {
             const std::pair<int, int> fk(in.Px, in.tyf);
             if (in.zf) {
@@ -559,7 +586,7 @@ void build_lists(mmc_ctx *c)
             fs.lds = std::max(fs.lds, (3 * (size_t)in.tyf + 1) * in.Px * cs);
             fs.bytes += inst_bytes(c, in, K_XINVM);
         }
-        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl})
+        for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl, &c->rl})
             l->first.push_back(l->n());
     }
     c->zls.lds = c->zl.lds;
@@ -571,6 +598,14 @@ void build_lists(mmc_ctx *c)
         sg.count = (int)kv.second.size();
         c->fl.host.insert(c->fl.host.end(), kv.second.begin(), kv.second.end());
         c->fl.seg.push_back(sg);
+    }
+    for (auto &kv : ritems) {
+        xcd_group(kv.second, line_share(2, cs));
+        Segment sg = rclass[kv.first];
+        sg.begin = c->rl.n();
+        sg.count = (int)kv.second.size();
+        c->rl.host.insert(c->rl.host.end(), kv.second.begin(), kv.second.end());
+        c->rl.seg.push_back(sg);
     }
     // x and y passes: one contiguous segment (and launch) per kernel variant
     for (auto &kv : xitems) {
@@ -741,6 +776,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_m, std::max<size_t>(1, m) * ts));
     HIPCHK(c, hipMalloc(&c->d_h, std::max<size_t>(1, m) * ts));
     HIPCHK(c, hipMemsetAsync(c->d_h, 0, std::max<size_t>(1, m) * ts, c->stream));
+    if (c->rows_ok) HIPCHK(c, hipMalloc(&c->d_m2, std::max<size_t>(1, m) * ts));
     if (a + b) HIPCHK(c, hipMalloc(&c->d_a, (a + b) * cs));
     c->d_b = c->d_a;
     if (kh) HIPCHK(c, hipMalloc(&c->d_khat, kh * cs));
@@ -765,7 +801,7 @@ int alloc_all(mmc_ctx *c)
     HIPCHK(c, hipMalloc(&c->d_H, sizeof(double) * 3 * std::max<long>(1, c->maxN)));
     HIPCHK(c, hipMalloc(&c->dInst, sizeof(Inst) * c->count));
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl, &c->rl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -840,7 +876,7 @@ int retile(mmc_ctx *c, int tyx, int ncol)
         }
     }
     build_lists(c);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl}) {
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl, &c->rl}) {
         int rc = upload_list(c, *l);
         if (rc) return rc;
     }
@@ -998,6 +1034,18 @@ template <typename T>
 int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
 {
     hipStream_t st = c->stream;
+    if (fused_sweeps(c) && c->rows) {
+        // sweep s of a run reads state buffer s & 1 and writes the other
+        int rc = pass_zyz<T>(c, true);
+        if (rc) return rc;
+        const Bufs<T> b(c);
+        const T *m_in = (const T *)((sweep_off & 1) ? c->d_m2 : c->d_m);
+        T *m_out = (T *)((sweep_off & 1) ? c->d_m : c->d_m2);
+        return run_segments(c, c->rl, K_ROWS, [&](const Segment &sg, hipStream_t s) {
+            return launch_row_sweep<T>(s, c->dInst, c->rl.dev + sg.begin, sg.count, sg.fixed_n, b.A,
+                                       m_in, m_out, c->d_beta, c->d_ctl, sweep_off, b.tw, sg.lds);
+        });
+    }
     if (fused_sweeps(c)) {
         int rc = pass_zyz<T>(c, true);
         if (rc) return rc;
@@ -1048,15 +1096,24 @@ int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
         if (!rc && c->zls.n() != c->zl.n()) rc = pass_zfwd<T>(c, false);
         if (rc) return rc;
     }
+    // row sweeps alternate two state buffers: an odd run ends in the second
+    const bool pingpong = fused_sweeps(c) && c->rows;
+    auto settle = [&](uint32_t eager) {
+        if (pingpong && (eager & 1))
+            return hipMemcpyAsync(c->d_m, c->d_m2, c->m_elems * c->tsize, hipMemcpyDeviceToDevice, st);
+        return hipSuccess;
+    };
     if (!c->use_graph || c->timing) {
         HIPCHK(c, launch_set_ctl(st, c->d_ctl, stage, sweep0));
         for (uint32_t s = 0; s < count; ++s) {
             int rc = enqueue_sweep<T>(c, s);
             if (rc) return rc;
         }
+        HIPCHK(c, settle(count));
         return MMC_OK;
     }
-    const int G = c->graph_sweeps > 0 ? c->graph_sweeps : 16;
+    int G = c->graph_sweeps > 0 ? c->graph_sweeps : 16;
+    if (pingpong) G += G & 1;   // a graph of row sweeps starts and ends in the first buffer
     if (!c->gexec) {
         HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = MMC_OK;
@@ -1080,6 +1137,7 @@ int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
             int rc = enqueue_sweep<T>(c, k);
             if (rc) return rc;
         }
+        HIPCHK(c, settle(count - s));
     }
     return MMC_OK;
 }
@@ -1283,6 +1341,7 @@ struct Schedule {
     int tiles;          // 0 chosen for the launch sizes, 1 minimal, 2 widest
     bool zfold;         // fused tiles of short-z instances fold z in
     bool conc;          // per-length launches of a pass on side streams
+    bool rows;          // fused sweeps as the y pass + whole-sweep row-pair tiles
 };
 
 // Fused tiles that fold z in where an instance admits them (zfold), else
@@ -1316,6 +1375,7 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
     c->fuse = s.fuse;
     c->metro_tiled = s.tiled;
     c->concurrent = s.conc;
+    c->rows = s.rows && c->rows_ok;
     for (auto &in : c->inst) in.zf = 0;
     set_zfold(c, s.zfold);
     return retile(c, 0, 0);
@@ -1349,18 +1409,21 @@ int time_sweeps(mmc_ctx *c, int sweeps, double *us)
 }
 
 const Schedule kSchedules[] = {
-        {"fused, launch-sized tiles", true, false, 0, false, false},
-        {"fused, minimal tiles", true, false, 1, false, false},
-        {"fused, widest tiles", true, false, 2, false, false},
-        {"fused, z folded", true, false, 0, true, false},
-        {"separate, launch-sized tiles", false, false, 0, false, false},
-        {"separate, minimal tiles", false, false, 1, false, false},
-        {"separate, widest tiles", false, false, 2, false, false},
-        {"tiled two-colour Metropolis", false, true, 0, false, false},
-        {"fused, launch-sized, concurrent", true, false, 0, false, true},
-        {"fused, minimal, concurrent", true, false, 1, false, true},
-        {"separate, launch-sized, concurrent", false, false, 0, false, true},
-        {"separate, minimal, concurrent", false, false, 1, false, true},
+        {"fused, launch-sized tiles", true, false, 0, false, false, false},
+        {"fused, minimal tiles", true, false, 1, false, false, false},
+        {"fused, widest tiles", true, false, 2, false, false, false},
+        {"fused, z folded", true, false, 0, true, false, false},
+        {"separate, launch-sized tiles", false, false, 0, false, false, false},
+        {"separate, minimal tiles", false, false, 1, false, false, false},
+        {"separate, widest tiles", false, false, 2, false, false, false},
+        {"tiled two-colour Metropolis", false, true, 0, false, false, false},
+        {"fused, launch-sized, concurrent", true, false, 0, false, true, false},
+        {"fused, minimal, concurrent", true, false, 1, false, true, false},
+        {"separate, launch-sized, concurrent", false, false, 0, false, true, false},
+        {"separate, minimal, concurrent", false, false, 1, false, true, false},
+        {"row sweeps, launch-sized tiles", true, false, 0, false, false, true},
+        {"row sweeps, minimal tiles", true, false, 1, false, false, true},
+        {"row sweeps, minimal, concurrent", true, false, 1, false, true, true},
 };
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
@@ -1389,7 +1452,7 @@ bool schedule_useful(mmc_ctx *c, const Schedule &s)
         c->zfold = z;
     }
     return !((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
-             (s.conc && !multi));
+             (s.conc && !multi) || (s.rows && !c->rows_ok));
 }
 
 template <typename T>
@@ -1454,11 +1517,11 @@ void mmc_destroy(mmc_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-    for (void *p : {c->d_m, c->d_h, c->d_a, c->d_khat, (void *)c->d_K, c->d_tw32, c->d_tw64,
+    for (void *p : {c->d_m, c->d_m2, c->d_h, c->d_a, c->d_khat, (void *)c->d_K, c->d_tw32, c->d_tw64,
                     (void *)c->d_beta, (void *)c->d_ctl, (void *)c->d_part, (void *)c->d_eout,
                     (void *)c->d_efirst, (void *)c->d_H, (void *)c->dInst})
         if (p) (void)hipFree(p);
-    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl})
+    for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl, &c->rl})
         if (l->dev) (void)hipFree(l->dev);
     for (auto s : c->side) (void)hipStreamDestroy(s);
     for (auto e : c->join_ev) (void)hipEventDestroy(e);
@@ -1904,6 +1967,15 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
             return fail(c, MMC_EUNSUP, "fused x-pass + Metropolis sweeps need demag, a "
                                        "checkerboard and a fixed-plan x length for every instance");
         c->fuse = value != 0;
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+        return MMC_OK;
+    }
+    if (k == "rows") {
+        if (value && !c->rows_ok)
+            return fail(c, MMC_EUNSUP, "row sweeps need single-layer checkerboard films with "
+                                       "demag and a fused-kernel x length for every instance");
+        c->rows = value != 0;
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
         return MMC_OK;

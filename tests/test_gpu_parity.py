@@ -534,3 +534,48 @@ def test_few_layers_sum_z_directly(orc, mmc, geo, precision):
         assert np.array_equal(h0, out[0][0])
         assert np.array_equal(m, out[0][1])
         assert e == out[0][2]
+
+
+@pytest.mark.parametrize("geos", [[dict(nx=64, ny=40)], [dict(nx=128, ny=10, pbc_x=1, pbc_y=1)],
+                                  [dict(nx=64, ny=37)], [dict(nx=128, ny=2, pbc_y=1)],
+                                  [dict(nx=256, ny=1)],
+                                  [dict(nx=64, ny=64, seed=1), dict(nx=96, ny=31, seed=2),
+                                   dict(nx=128, ny=16, pbc_x=1, seed=3)]],
+                         ids=["open", "pbc", "odd_rows", "two_rows_pbc", "one_row", "batch"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_row_sweeps_match_fused(mmc, geos, precision):
+    """Whole-sweep row-pair tiles (halo pairs inverted and their colour-0
+    updates recomputed in each tile, state in two alternating buffers) give
+    the bit-identical chain and energies of the fused and separate
+    schedules: eager with an odd sweep count, graph-replayed with an odd
+    graph length, and after a further odd run."""
+    kws = [film(**g) for g in geos]
+    out = []
+    for fuse, rows, graph in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)):
+        c = mmc.Context([mmc.make_params(**kw) for kw in kws], precision)
+        c.build_tensor()
+        c.init_random_m()
+        c.set_temperature(650.0)
+        c.set_option("fuse", fuse)
+        c.set_option("rows", rows)
+        if rows:
+            assert c.kernel_bytes("row_sweep") > 0
+        if graph:
+            c.set_option("graph", 1)
+            c.set_option("graph_sweeps", 3)
+        c.sweeps(3, 0, 7)
+        m1 = [c.m(i) for i in range(len(kws))]
+        c.sweeps(3, 7, 5)
+        out.append((m1, [c.m(i) for i in range(len(kws))], c.energy()["total"]))
+    for m1, m2, e in out[1:]:
+        for i in range(len(kws)):
+            assert np.array_equal(m1[i], out[0][0][i]), f"instance {i}"
+            assert np.array_equal(m2[i], out[0][1][i]), f"instance {i}"
+        assert e == out[0][2]
+
+
+def test_row_sweeps_need_single_layer_films(mmc):
+    """The row schedule is refused where an instance has several layers."""
+    c = mmc.Context(mmc.make_params(**film(nx=64, ny=32, nz=2)), 1)
+    with pytest.raises(mmc.MMCError):
+        c.set_option("rows", 1)
