@@ -232,14 +232,14 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * pass; 0: separate launches; "metro_tiled" = 1 takes precedence),
  * "fuse_rows" (1: one pair of real rows per fused workgroup; >1: the full
  * tile, 2048 FP32 / 1024 FP64 complex per component over the x length),
- * "rows" (1, with "fuse": every instance being a single-layer checkerboard
- * film with a fused-kernel x length, a sweep is the y pass and one launch
- * of whole-sweep row-pair tiles, each inverting the x spectrum of its pair
- * and both neighbouring pairs, updating colour 0 on four rows (the halo
- * rows' updates recomputed as their own tiles compute them) and colour 1 on
- * its pair, and transforming the pair forward; the state alternates between
- * two device buffers, settled into the first at the end of every call;
- * default 0),
+ * "rows" (1, with "fuse": every instance being a checkerboard film of one
+ * or two layers with a fused-kernel x length and a tile that fits LDS, a
+ * sweep is the y pass and one launch of whole-sweep row-pair tiles, each
+ * inverting the x spectrum of its pair and both neighbouring pairs in every
+ * layer, updating colour 0 on four rows (the halo rows' updates recomputed
+ * as their own tiles compute them) and colour 1 on its pair, and
+ * transforming the pair forward; the state alternates between two device
+ * buffers, settled into the first at the end of every call; default 0),
  * "concurrent" (1: the per-length launches of a pass over a batch run on
  * side streams forked from and joined back into the context's stream;
  * default 0),

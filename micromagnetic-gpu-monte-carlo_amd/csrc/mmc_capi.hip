@@ -385,8 +385,8 @@ double inst_bytes(const mmc_ctx *c, const Inst &in, int k)
     // read (metro_xfwd)
     case K_XINVM: case K_MXFWD:
         return in.cal_demag ? (in.zf ? spec_z : spec_x) + 6 * N * b : 0.0;
-    // whole sweep of row pairs: spectrum read and written, m read (cells and
-    // neighbours) and written
+    // whole sweep of row pairs (all layers): spectrum read and written, m
+    // read (cells and neighbours) and written
     case K_ROWS: return in.cal_demag ? 2 * spec_x + 6 * N * b : 0.0;
     case K_TENSOR: return 6.0 * in.NP * 8;
     }
@@ -430,13 +430,20 @@ int fused_full(const mmc_ctx *c, Inst &in)
     return (4 * (size_t)in.Px * cs <= c->lds_max) ? 1 : 0;
 }
 
+// LDS of a whole-sweep row-pair tile: three row pairs of every layer and
+// component, and the twiddles.
+size_t row_lds(const mmc_ctx *c, const Inst &in)
+{
+    return (9 * (size_t)in.nz + 1) * in.Px * 2 * c->tsize;
+}
+
 // Can instance `in` take whole-sweep row-pair tiles (k_row_sweep)?  A
-// single-layer checkerboard film with demag, a fused-kernel x length and
-// room for three row pairs and the twiddles in LDS.
+// checkerboard film of one or two layers with demag, a fused-kernel x
+// length and room for its tile in LDS.
 bool row_ok(const mmc_ctx *c, const Inst &in)
 {
-    return in.cal_demag && in.nz == 1 && in.ncolors == 2 && xfixed_size(in.Px) &&
-           10 * (size_t)in.Px * 2 * c->tsize <= c->lds_max;
+    return in.cal_demag && in.nz <= 2 && in.ncolors == 2 && xfixed_size(in.Px) &&
+           row_lds(c, in) <= c->lds_max;
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by index, and each
@@ -478,8 +485,8 @@ void build_lists(mmc_ctx *c)
     std::map<std::pair<int, int>, std::vector<Work>> yitems;
     std::map<std::pair<int, int>, Segment> xclass;   // (fixed length, rows) -> items
     std::map<std::pair<int, int>, std::vector<Work>> xitems;
-    std::map<int, Segment> rclass;                   // x length -> row-sweep items
-    std::map<int, std::vector<Work>> ritems;
+    std::map<std::pair<int, int>, Segment> rclass;   // (x length, layers) -> row-sweep items
+    std::map<std::pair<int, int>, std::vector<Work>> ritems;
     c->rows_ok = !c->inst.empty();
     for (auto &in : c->inst) c->rows_ok = c->rows_ok && row_ok(c, in);
     std::map<std::pair<int, int>, Segment> fclass;   // (x length, row pairs) -> items
@@ -565,10 +572,12 @@ void build_lists(mmc_ctx *c)
             c->mtl.lds = std::max(c->mtl.lds, mtile_lds(c, in.mtx, in.mty, in.nz));
         }
         if (c->rows_ok) {
-            for (int y0 = 0; y0 < in.ny; y0 += 2) ritems[in.Px].push_back({i, 0, 0, y0});
-            Segment &rs = rclass[in.Px];
+            const std::pair<int, int> rk(in.Px, in.nz);
+            for (int y0 = 0; y0 < in.ny; y0 += 2) ritems[rk].push_back({i, 0, 0, y0});
+            Segment &rs = rclass[rk];
             rs.fixed_n = in.Px;
-            rs.lds = std::max(rs.lds, 10 * (size_t)in.Px * cs);
+            rs.rows = in.nz;
+            rs.lds = std::max(rs.lds, row_lds(c, in));
             rs.bytes += inst_bytes(c, in, K_ROWS);
         }
         if (in.tyf) {
@@ -1042,8 +1051,9 @@ int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
         const T *m_in = (const T *)((sweep_off & 1) ? c->d_m2 : c->d_m);
         T *m_out = (T *)((sweep_off & 1) ? c->d_m : c->d_m2);
         return run_segments(c, c->rl, K_ROWS, [&](const Segment &sg, hipStream_t s) {
-            return launch_row_sweep<T>(s, c->dInst, c->rl.dev + sg.begin, sg.count, sg.fixed_n, b.A,
-                                       m_in, m_out, c->d_beta, c->d_ctl, sweep_off, b.tw, sg.lds);
+            return launch_row_sweep<T>(s, c->dInst, c->rl.dev + sg.begin, sg.count, sg.fixed_n,
+                                       sg.rows, b.A, m_in, m_out, c->d_beta, c->d_ctl, sweep_off,
+                                       b.tw, sg.lds);
         });
     }
     if (fused_sweeps(c)) {
@@ -1973,8 +1983,9 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
     }
     if (k == "rows") {
         if (value && !c->rows_ok)
-            return fail(c, MMC_EUNSUP, "row sweeps need single-layer checkerboard films with "
-                                       "demag and a fused-kernel x length for every instance");
+            return fail(c, MMC_EUNSUP, "row sweeps need checkerboard films of one or two layers "
+                                       "with demag, a fused-kernel x length and room for the "
+                                       "tile in LDS for every instance");
         c->rows = value != 0;
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }

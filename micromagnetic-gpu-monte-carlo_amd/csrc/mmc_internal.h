@@ -153,17 +153,18 @@ hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n
                              cpx<T> *spec, const double *beta,
                              const uint32_t *ctl, uint32_t sweep_off,
                              const cpx<T> *tw, size_t lds);
-// Threads of a k_row_sweep workgroup: a tile alone on its CU for small films
-// (one tile per row pair), so its Metropolis cells and butterflies are
-// spread over sixteen waves to keep their dependent latencies short.
+// Threads of a k_row_sweep workgroup: a tile is alone on its CU (small films
+// have fewer tiles than CUs; large ones fill its LDS), so its Metropolis
+// cells and butterflies are spread over sixteen waves to keep their
+// dependent latencies short.
 constexpr int kRowThreads = 1024;
-// A whole sweep of row pairs of single-layer checkerboard films (x length
-// fixed_n, a fused-kernel length): items (inst, 0, 0, y0), y0 even; reads
-// the pre-sweep state m_in and the x spectrum, writes the pairs' final
-// state to m_out and their x spectrum back to spec.
+// A whole sweep of row pairs, all layers, of checkerboard films of nz = 1 or
+// 2 layers (x length fixed_n, a fused-kernel length): items (inst, 0, 0,
+// y0), y0 even; reads the pre-sweep state m_in and the x spectrum, writes
+// the pairs' final state to m_out and their x spectrum back to spec.
 template <typename T>
 hipError_t launch_row_sweep(hipStream_t st, const Inst *I, const Work *W, int n,
-                            int fixed_n, cpx<T> *spec, const T *m_in, T *m_out,
+                            int fixed_n, int nz, cpx<T> *spec, const T *m_in, T *m_out,
                             const double *beta, const uint32_t *ctl,
                             uint32_t sweep_off, const cpx<T> *tw, size_t lds);
 template <typename T>

@@ -16,8 +16,8 @@
 //                   src/mc_sampling.h:27-41)
 //   k_xinv_metro    inverse x pass fused with the first checkerboard colour
 //   k_metro_xfwd    second checkerboard colour fused with the forward x pass
-//   k_row_sweep     a whole sweep of a row pair of a single-layer film: inverse
-//                   x pass (with halo pairs), both colours, forward x pass
+//   k_row_sweep     a whole sweep of a row pair of a film of one or two layers:
+//                   inverse x pass (with halo pairs), both colours, forward x pass
 //   k_energy(+final) per-cell energy terms and <m>, compensated FP64
 //                   block partials, fixed-order final sum (src/main.cu:945-947)
 //   k_field         effective field in A/m (effectiveHKernel,
@@ -1787,34 +1787,36 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
     }
 }
 
-// One whole Metropolis sweep of a pair of real rows (y0, y0+1) of a
-// single-layer checkerboard film in one launch (with the y pass, two
-// launches per sweep instead of three).  The tile inverts the x spectrum of
-// its pair and of both neighbouring pairs (pairs start at even rows, as in
-// their own tiles, so the arithmetic is theirs), updates colour 0 on rows
-// y0-1..y0+2 (the halo rows' updates recomputed exactly as their own tiles
-// compute them), colour 1 on its pair, and transforms the pair's final
-// state forward into A.  Every read of m is of the pre-sweep state m_in,
-// which no workgroup writes; the pair's final state goes to m_out.  LDS row
-// (p, comp) = p*3 + comp holds pair p = 0, 1, 2 <-> rows y0-2.., y0.., y0+2..
-// (two real rows packed per complex row); after the inverse transform a
-// colour-0 cell's slot is overwritten with its new value, then a colour-1
-// cell's.  Same arithmetic as k_xinv_metro + k_metro_xfwd with one row pair.
-template <typename T, int N>
+// One whole Metropolis sweep of a pair of real rows (y0, y0+1), in every one
+// of the NZ layers, of a checkerboard film in one launch (with the y pass,
+// two launches per sweep instead of three).  The tile inverts the x spectrum
+// of its pair and of both neighbouring pairs in every layer (pairs start at
+// even rows, as in their own tiles, so the arithmetic is theirs), updates
+// colour 0 on rows y0-1..y0+2 (the halo rows' updates recomputed exactly as
+// their own tiles compute them), colour 1 on its pair, and transforms the
+// pair's final state forward into A.  Every read of m is of the pre-sweep
+// state m_in, which no workgroup writes; the pair's final state goes to
+// m_out.  LDS row ((p*NZ + z)*3 + comp) holds layer z of pair p = 0, 1, 2 <->
+// rows y0-2.., y0.., y0+2.. (two real rows packed per complex row); after
+// the inverse transform a colour-0 cell's slot is overwritten with its new
+// value, then a colour-1 cell's.  Same arithmetic as k_xinv_metro +
+// k_metro_xfwd with one row pair per layer.
+template <typename T, int N, int NZ>
 __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restrict__ I,
-                                                        const Work *__restrict__ W,
-                                                        cpx<T> *__restrict__ specb,
-                                                        const T *__restrict__ mib,
-                                                        T *__restrict__ mob,
-                                                        const double *__restrict__ beta,
-                                                        const uint32_t *__restrict__ ctl,
-                                                        uint32_t sweep_off,
-                                                        const cpx<T> *__restrict__ twb)
+                                                           const Work *__restrict__ W,
+                                                           cpx<T> *__restrict__ specb,
+                                                           const T *__restrict__ mib,
+                                                           T *__restrict__ mob,
+                                                           const double *__restrict__ beta,
+                                                           const uint32_t *__restrict__ ctl,
+                                                           uint32_t sweep_off,
+                                                           const cpx<T> *__restrict__ twb)
 {
     constexpr int H = N / 2 + 1;
     constexpr int NT = kRowThreads;
+    constexpr int PR = 3 * NZ * 3;                 // LDS rows: pairs x layers x components
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
-    cpx<T> *tw = s + 9 * N;
+    cpx<T> *tw = s + PR * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
     const int y0 = w.c;
@@ -1824,7 +1826,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     const T *mi_ = mib + in.m_off;
     T *mo = mob + in.m_off;
     cpx<T> *A = specb + in.a_off;
-    const long cstride = (long)H * ny;          // component stride in A (nz = 1)
+    const long cstride = (long)H * NZ * ny;     // component stride in A
     const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
     const T bt = T(beta[w.inst]);
     // absolute row of tile-relative row r (-2..3), or -1 outside an open axis
@@ -1836,130 +1838,137 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         }
         return y;
     };
+    auto cell = [&](int y, int z, int x) { return (long)nx * (y + (long)ny * z) + x; };
     stage_twiddles<T, N, NT>(tw, twb + in.twx);
     // Cells of this thread: colour 0 on rows -1..2 (QC0 at most, CH0 per
-    // gather) and colour 1 on rows 0, 1 (QC1 at most).  Their inputs from
-    // m_in and their trial moves depend on nothing this kernel computes, so
-    // for short rows (PF) they are all fetched before the transform, behind
-    // the spectrum loads.
-    constexpr int QC0 = (4 * (N / 2) + NT - 1) / NT;
-    constexpr int QC1 = (2 * (N / 2) + NT - 1) / NT;
+    // gather) and colour 1 on rows 0, 1 (QC1 at most), all layers.  Their
+    // inputs from m_in and their trial moves depend on nothing this kernel
+    // computes, so for short rows (PF) they are all fetched before the
+    // transform, behind the spectrum loads.
+    constexpr int QC0 = (4 * NZ * (N / 2) + NT - 1) / NT;
+    constexpr int QC1 = (2 * NZ * (N / 2) + NT - 1) / NT;
     constexpr bool PF = sizeof(T) == 4 && QC0 <= 2 && QC1 <= 1;
     constexpr int CH0 = PF ? QC0 : (QC0 < gather_chunk<T>() ? QC0 : gather_chunk<T>());
     constexpr int CH1 = PF ? QC1 : (QC1 < gather_chunk<T>() ? QC1 : gather_chunk<T>());
-    const int ncell0 = 4 * hw, ncell1 = 2 * hw;
-    auto cell0 = [&](int q, int &r, int &y, int &x) {   // y < 0: no cell
+    const int ncell0 = 4 * NZ * hw, ncell1 = 2 * NZ * hw;
+    // cell e = ((r - r0)*NZ + z)*hw + j; y < 0: no cell
+    auto cell0 = [&](int q, int &r, int &z, int &y, int &x) {
         const int e = threadIdx.x + q * NT;
-        r = e / hw - 1;
-        const int j = e - (r + 1) * hw;
+        const int rz = e / hw, j = e - rz * hw;
+        r = rz / NZ - 1;
+        z = rz - (r + 1) * NZ;
         y = q < QC0 && e < ncell0 ? row_of(r) : -1;
-        x = 2 * j + (y & 1);
+        x = 2 * j + ((y + z) & 1);
         if (x >= nx) y = -1;
     };
-    auto cell1 = [&](int q, int &r, int &y, int &x) {
+    auto cell1 = [&](int q, int &r, int &z, int &y, int &x) {
         const int e = threadIdx.x + q * NT;
-        r = e / hw;
-        const int j = e - r * hw;
+        const int rz = e / hw, j = e - rz * hw;
+        r = rz / NZ;
+        z = rz - r * NZ;
         y = q < QC1 && e < ncell1 ? row_of(r) : -1;
-        x = 2 * j + ((y + 1) & 1);
+        x = 2 * j + ((y + z + 1) & 1);
         if (x >= nx) y = -1;
     };
     V3<T> c0mi[CH0], c1mi[CH1];
     Nbrs<T> c0nb[CH0];
     Move<T> c0mv[CH0], c1mv[CH1];
     auto fetch0 = [&](int q, int k) {
-        int r, y, x;
-        cell0(q, r, y, x);
+        int r, z, y, x;
+        cell0(q, r, z, y, x);
         if (y >= 0) {
-            const long i = (long)nx * y + x;
+            const long i = cell(y, z, x);
             c0mi[k] = load_m(mi_, NN, i);
-            c0nb[k] = gather_nbrs(in, mi_, x, y, 0);
+            c0nb[k] = gather_nbrs(in, mi_, x, y, z);
             c0mv[k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
         }
     };
     auto fetch1 = [&](int q, int k) {
-        int r, y, x;
-        cell1(q, r, y, x);
+        int r, z, y, x;
+        cell1(q, r, z, y, x);
         if (y >= 0) {
-            const long i = (long)nx * y + x;
+            const long i = cell(y, z, x);
             c1mi[k] = load_m(mi_, NN, i);
             c1mv[k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
         }
     };
-    // spectrum of the three pairs, stored conjugated for the inverse
-    // transform (as k_xinv_metro); a pair outside an open axis stays zero
-    const int per = 3 * H;
+    // spectrum of the three pairs of every layer, stored conjugated for the
+    // inverse transform (as k_xinv_metro); a pair outside an open axis stays
+    // zero.  Element e = ((p*NZ + z)*3 + c)*H + k.
+    constexpr int PER = PR * H;
     auto spec_ld = [&](int e) {
-            const int p = e / per, r = e - p * per;
-            const int c = r / H, k = r - c * H;
-            const int ya = row_of(2 * (p - 1)), yb = row_of(2 * (p - 1) + 1);
-            Pair2<cpx<T>> v;
-            T ax = T(0), ay = T(0), bx = T(0), by = T(0);
-            if (ya >= 0) {
-                const cpx<T> a = A[c * cstride + (long)k * ny + ya];
-                ax = a.x;
-                ay = a.y;
-                if (yb >= 0) {
-                    const cpx<T> b = A[c * cstride + (long)k * ny + yb];
-                    bx = b.x;
-                    by = b.y;
-                }
+        const int rr = e / H, k = e - rr * H;
+        const int pz = rr / 3, p = pz / NZ, z = pz - p * NZ, c = rr - 3 * pz;
+        const int ya = row_of(2 * (p - 1)), yb = row_of(2 * (p - 1) + 1);
+        Pair2<cpx<T>> v;
+        T ax = T(0), ay = T(0), bx = T(0), by = T(0);
+        if (ya >= 0) {
+            const cpx<T> *col = A + c * cstride + ((long)k * NZ + z) * ny;
+            const cpx<T> a = col[ya];
+            ax = a.x;
+            ay = a.y;
+            if (yb >= 0) {
+                const cpx<T> b = col[yb];
+                bx = b.x;
+                by = b.y;
             }
-            v.a = {ax, ay};
-            v.b = {bx, by};
-            return v;
+        }
+        v.a = {ax, ay};
+        v.b = {bx, by};
+        return v;
     };
     auto spec_st = [&](int e, Pair2<cpx<T>> v) {
-            const int p = e / per, r = e - p * per;
-            const int c = r / H, k = r - c * H;
-            const cpx<T> a = v.a;
-            const T bx = v.b.x, by = v.b.y;
-            cpx<T> *row = s + (p * 3 + c) * N;
-            row[k] = {a.x - by, -(a.y + bx)};
-            if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
+        const int rr = e / H, k = e - rr * H;
+        const cpx<T> a = v.a;
+        const T bx = v.b.x, by = v.b.y;
+        cpx<T> *row = s + rr * N;
+        row[k] = {a.x - by, -(a.y + bx)};
+        if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
     };
     if (PF) {
-        const auto spec = staged_load<9 * H, NT, Pair2<cpx<T>>>(3 * per, spec_ld);
+        const auto spec = staged_load<PER, NT, Pair2<cpx<T>>>(PER, spec_ld);
         #pragma unroll
         for (int q = 0; q < QC0; ++q) fetch0(q, q);
         #pragma unroll
         for (int q = 0; q < QC1; ++q) fetch1(q, q);
-        staged_store<9 * H, NT, Pair2<cpx<T>>>(3 * per, spec, spec_st);
+        staged_store<PER, NT, Pair2<cpx<T>>>(PER, spec, spec_st);
     } else {
-        staged_copy<9 * H, NT, Pair2<cpx<T>>>(3 * per, spec_ld, spec_st);
+        staged_copy<PER, NT, Pair2<cpx<T>>>(PER, spec_ld, spec_st);
     }
     __syncthreads();
-    fft_fixed<T, N, 9, NT>(s, tw);
+    fft_fixed<T, N, PR, NT>(s, tw);
     // the pair's slots outside the film are zero for the forward transform
     // (no cell reads them: x >= nx, or a row past an open axis)
     const int nyt = min(2, ny - y0);
-    for (int e = threadIdx.x; e < 3 * N; e += NT) {
-        const int c = e / N, x = e - c * N;
-        cpx<T> &v = s[(3 + c) * N + x];
+    for (int e = threadIdx.x; e < 3 * NZ * N; e += NT) {
+        const int x = e % N;
+        cpx<T> &v = s[3 * NZ * N + e];
         if (x >= nx) v = {T(0), T(0)};
         else if (nyt < 2) v.y = T(0);
     }
-    // slot of component c at tile-relative row r (-2..3), column x; an odd
-    // row sits in the imaginary part (its demag value negated)
-    auto slot = [&](int r, int c, int x) -> cpx<T> & { return s[(((r + 2) >> 1) * 3 + c) * N + x]; };
-    auto hval = [&](int r, int x) -> V3<T> {
-        if ((r + 2) & 1) return {-slot(r, 0, x).y, -slot(r, 1, x).y, -slot(r, 2, x).y};
-        return {slot(r, 0, x).x, slot(r, 1, x).x, slot(r, 2, x).x};
+    // slot of component c at tile-relative row r (-2..3), layer z, column x;
+    // an odd row sits in the imaginary part (its demag value negated)
+    auto slot = [&](int r, int z, int c, int x) -> cpx<T> & {
+        return s[((((r + 2) >> 1) * NZ + z) * 3 + c) * N + x];
     };
-    auto put = [&](int r, int x, const V3<T> &v) {
+    auto hval = [&](int r, int z, int x) -> V3<T> {
+        if ((r + 2) & 1) return {-slot(r, z, 0, x).y, -slot(r, z, 1, x).y, -slot(r, z, 2, x).y};
+        return {slot(r, z, 0, x).x, slot(r, z, 1, x).x, slot(r, z, 2, x).x};
+    };
+    auto put = [&](int r, int z, int x, const V3<T> &v) {
         if ((r + 2) & 1) {
-            slot(r, 0, x).y = v.x;
-            slot(r, 1, x).y = v.y;
-            slot(r, 2, x).y = v.z;
+            slot(r, z, 0, x).y = v.x;
+            slot(r, z, 1, x).y = v.y;
+            slot(r, z, 2, x).y = v.z;
         } else {
-            slot(r, 0, x).x = v.x;
-            slot(r, 1, x).x = v.y;
-            slot(r, 2, x).x = v.z;
+            slot(r, z, 0, x).x = v.x;
+            slot(r, z, 1, x).x = v.y;
+            slot(r, z, 2, x).x = v.z;
         }
     };
-    auto mval = [&](int r, int x) -> V3<T> {   // new value written by put()
-        if ((r + 2) & 1) return {slot(r, 0, x).y, slot(r, 1, x).y, slot(r, 2, x).y};
-        return {slot(r, 0, x).x, slot(r, 1, x).x, slot(r, 2, x).x};
+    auto mval = [&](int r, int z, int x) -> V3<T> {   // new value written by put()
+        if ((r + 2) & 1) return {slot(r, z, 0, x).y, slot(r, z, 1, x).y, slot(r, z, 2, x).y};
+        return {slot(r, z, 0, x).x, slot(r, z, 1, x).x, slot(r, z, 2, x).x};
     };
     // colour 0 on rows -1..2 (neighbours: pre-sweep colour-1 cells of m_in)
     #pragma unroll
@@ -1970,18 +1979,18 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         }
         #pragma unroll
         for (int k = 0; k < CH0; ++k) {
-            int r, y, x;
-            cell0(q0 + k, r, y, x);
+            int r, z, y, x;
+            cell0(q0 + k, r, z, y, x);
             if (y < 0) continue;
             V3<T> ex, dmi;
             bond_sums(in, c0nb[k].xp, c0nb[k].xm, c0nb[k].yp, c0nb[k].ym, c0nb[k].zp, c0nb[k].zm,
                       ex, dmi);
-            const V3<T> mn = metro_accept(in, c0mi[k], ex, dmi, hval(r, x), 1, c0mv[k], bt)
+            const V3<T> mn = metro_accept(in, c0mi[k], ex, dmi, hval(r, z, x), 1, c0mv[k], bt)
                                  ? c0mv[k].mn
                                  : c0mi[k];
-            put(r, x, mn);
+            put(r, z, x, mn);
             if (r == 0 || r == 1) {
-                const long i = (long)nx * y + x;
+                const long i = cell(y, z, x);
                 mo[i] = mn.x;
                 mo[NN + i] = mn.y;
                 mo[2 * NN + i] = mn.z;
@@ -2000,37 +2009,42 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
             }
             #pragma unroll
             for (int k = 0; k < CH1; ++k) {
-                int r, y, x;
-                cell1(q0 + k, r, y, x);
+                int r, z, y, x;
+                cell1(q0 + k, r, z, y, x);
                 if (y < 0) continue;
                 Nbrs<T> nb = {zero, zero, zero, zero, zero, zero};
                 int t;
-                if ((t = nbr(x, +1, nx, in.pbc[0])) >= 0) nb.xp = mval(r, t);
-                if ((t = nbr(x, -1, nx, in.pbc[0])) >= 0) nb.xm = mval(r, t);
-                if (nbr(y, +1, ny, pery) >= 0) nb.yp = mval(r + 1, x);
-                if (nbr(y, -1, ny, pery) >= 0) nb.ym = mval(r - 1, x);
+                if ((t = nbr(x, +1, nx, in.pbc[0])) >= 0) nb.xp = mval(r, z, t);
+                if ((t = nbr(x, -1, nx, in.pbc[0])) >= 0) nb.xm = mval(r, z, t);
+                if (nbr(y, +1, ny, pery) >= 0) nb.yp = mval(r + 1, z, x);
+                if (nbr(y, -1, ny, pery) >= 0) nb.ym = mval(r - 1, z, x);
+                if (NZ > 1) {
+                    if ((t = nbr(z, +1, NZ, in.pbc[2])) >= 0) nb.zp = mval(r, t, x);
+                    if ((t = nbr(z, -1, NZ, in.pbc[2])) >= 0) nb.zm = mval(r, t, x);
+                }
                 V3<T> ex, dmi;
                 bond_sums(in, nb.xp, nb.xm, nb.yp, nb.ym, nb.zp, nb.zm, ex, dmi);
-                const V3<T> mn = metro_accept(in, c1mi[k], ex, dmi, hval(r, x), 1, c1mv[k], bt)
+                const V3<T> mn = metro_accept(in, c1mi[k], ex, dmi, hval(r, z, x), 1, c1mv[k], bt)
                                      ? c1mv[k].mn
                                      : c1mi[k];
-                const long i = (long)nx * y + x;
+                const long i = cell(y, z, x);
                 mo[i] = mn.x;
                 mo[NN + i] = mn.y;
                 mo[2 * NN + i] = mn.z;
-                put(r, x, mn);
+                put(r, z, x, mn);
             }
         }
     }
     __syncthreads();
     // the pair's final state (zero outside the film) forward into A
-    fft_fixed<T, N, 3, NT>(s + 3 * N, tw);
-    for (int e = threadIdx.x; e < 3 * H; e += NT) {
-        const int c = e / H, k = e - c * H;
-        const cpx<T> *row = s + (3 + c) * N;
+    fft_fixed<T, N, 3 * NZ, NT>(s + 3 * NZ * N, tw);
+    for (int e = threadIdx.x; e < 3 * NZ * H; e += NT) {
+        const int rr = e / H, k = e - rr * H;
+        const int z = rr / 3, c = rr - 3 * z;
+        const cpx<T> *row = s + (3 * NZ + rr) * N;
         const cpx<T> Z = row[k];
         const cpx<T> B = row[k ? N - k : 0];
-        cpx<T> *o = A + c * cstride + (long)k * ny + y0;
+        cpx<T> *o = A + c * cstride + ((long)k * NZ + z) * ny + y0;
         o[0] = {(Z.x + B.x) * T(0.5), (Z.y - B.y) * T(0.5)};
         if (nyt > 1) o[1] = {(Z.y + B.y) * T(0.5), (B.x - Z.x) * T(0.5)};
     }
@@ -2472,21 +2486,22 @@ hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n
 }
 
 template <typename T>
-hipError_t launch_row_sweep(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
+hipError_t launch_row_sweep(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n, int nz,
                             cpx<T> *spec, const T *m_in, T *m_out, const double *beta,
                             const uint32_t *ctl, uint32_t sweep_off, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
     const dim3 g(n);
-#define MMC_RS_CASE(NN)                                                                          \
-    if (fixed_n == NN) {                                                                         \
-        MMC_LAUNCH((k_row_sweep<T, NN>), g, dim3(kRowThreads), lds, st, I, W, spec, m_in, m_out, \
-                   beta, ctl,                                                                    \
-                   sweep_off, tw);                                                               \
+#define MMC_RS_CASE(NN, ZZ)                                                                      \
+    if (fixed_n == NN && nz == ZZ) {                                                             \
+        MMC_LAUNCH((k_row_sweep<T, NN, ZZ>), g, dim3(kRowThreads), lds, st, I, W, spec, m_in,    \
+                   m_out, beta, ctl, sweep_off, tw);                                             \
         return post();                                                                           \
     }
-    MMC_RS_CASE(128) MMC_RS_CASE(192) MMC_RS_CASE(256) MMC_RS_CASE(384) MMC_RS_CASE(512)
-    MMC_RS_CASE(1024) MMC_RS_CASE(2048)
+#define MMC_RS_N(NN) MMC_RS_CASE(NN, 1) MMC_RS_CASE(NN, 2)
+    MMC_RS_N(128) MMC_RS_N(192) MMC_RS_N(256) MMC_RS_N(384) MMC_RS_N(512) MMC_RS_N(1024)
+    MMC_RS_N(2048)
+#undef MMC_RS_N
 #undef MMC_RS_CASE
     return hipErrorInvalidValue;
 }
@@ -2617,13 +2632,9 @@ static hipError_t set_lds_t(int bytes)
     MMC_YZL(768) MMC_YZL(1024) MMC_YZL(2048)
 #undef MMC_YZL
     constexpr int F = fused_rows_base<T>();
-    one((const void *)k_row_sweep<T, 128>);
-    one((const void *)k_row_sweep<T, 192>);
-    one((const void *)k_row_sweep<T, 256>);
-    one((const void *)k_row_sweep<T, 384>);
-    one((const void *)k_row_sweep<T, 512>);
-    one((const void *)k_row_sweep<T, 1024>);
-    one((const void *)k_row_sweep<T, 2048>);
+#define MMC_RSL(NN) one((const void *)k_row_sweep<T, NN, 1>); one((const void *)k_row_sweep<T, NN, 2>);
+    MMC_RSL(128) MMC_RSL(192) MMC_RSL(256) MMC_RSL(384) MMC_RSL(512) MMC_RSL(1024) MMC_RSL(2048)
+#undef MMC_RSL
 #define MMC_FL(TT, NN, RR) one((const void *)k_xinv_metro<TT, NN, RR>); one((const void *)k_metro_xfwd<TT, NN, RR>);
     MMC_FUSED(MMC_FL, T, F)
 #undef MMC_FL
@@ -2665,7 +2676,7 @@ hipError_t set_lds_limit(size_t bytes)
                                              const uint32_t *, uint32_t, const cpx<T> *,     \
                                              size_t);                                        \
     template hipError_t launch_row_sweep<T>(hipStream_t, const Inst *, const Work *, int, int,  \
-                                            cpx<T> *, const T *, T *, const double *,        \
+                                            int, cpx<T> *, const T *, T *, const double *,   \
                                             const uint32_t *, uint32_t, const cpx<T> *,      \
                                             size_t);                                         \
     template hipError_t launch_metro_tile<T>(hipStream_t, const Inst *, const Work *, int,    \

@@ -540,15 +540,19 @@ def test_few_layers_sum_z_directly(orc, mmc, geo, precision):
                                   [dict(nx=64, ny=37)], [dict(nx=128, ny=2, pbc_y=1)],
                                   [dict(nx=256, ny=1)],
                                   [dict(nx=64, ny=64, seed=1), dict(nx=96, ny=31, seed=2),
-                                   dict(nx=128, ny=16, pbc_x=1, seed=3)]],
-                         ids=["open", "pbc", "odd_rows", "two_rows_pbc", "one_row", "batch"])
+                                   dict(nx=128, ny=16, pbc_x=1, seed=3)],
+                                  [dict(nx=64, ny=32, nz=2)],
+                                  [dict(nx=128, ny=14, nz=2, pbc_y=1, pbc_z=1)],
+                                  [dict(nx=64, ny=21, nz=2, seed=4), dict(nx=64, ny=30, seed=5)]],
+                         ids=["open", "pbc", "odd_rows", "two_rows_pbc", "one_row", "batch",
+                              "two_layers", "two_layers_pbc", "mixed_layers"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_row_sweeps_match_fused(mmc, geos, precision):
     """Whole-sweep row-pair tiles (halo pairs inverted and their colour-0
-    updates recomputed in each tile, state in two alternating buffers) give
-    the bit-identical chain and energies of the fused and separate
-    schedules: eager with an odd sweep count, graph-replayed with an odd
-    graph length, and after a further odd run."""
+    updates recomputed in each tile, every layer in the tile, state in two
+    alternating buffers) give the bit-identical chain and energies of the
+    fused and separate schedules: eager with an odd sweep count,
+    graph-replayed with an odd graph length, and after a further odd run."""
     kws = [film(**g) for g in geos]
     out = []
     for fuse, rows, graph in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)):
@@ -574,8 +578,9 @@ def test_row_sweeps_match_fused(mmc, geos, precision):
         assert e == out[0][2]
 
 
-def test_row_sweeps_need_single_layer_films(mmc):
-    """The row schedule is refused where an instance has several layers."""
-    c = mmc.Context(mmc.make_params(**film(nx=64, ny=32, nz=2)), 1)
+def test_row_sweeps_need_thin_films(mmc):
+    """The row schedule is refused where an instance has more than two
+    layers."""
+    c = mmc.Context(mmc.make_params(**film(nx=64, ny=32, nz=3)), 1)
     with pytest.raises(mmc.MMCError):
         c.set_option("rows", 1)
