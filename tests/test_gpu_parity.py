@@ -584,3 +584,40 @@ def test_row_sweeps_need_thin_films(mmc):
     c = mmc.Context(mmc.make_params(**film(nx=64, ny=32, nz=3)), 1)
     with pytest.raises(mmc.MMCError):
         c.set_option("rows", 1)
+
+
+def test_row_sweeps_in_anneal(orc, mmc):
+    """The anneal driver under row sweeps (energies every 7 sweeps,
+    snapshots every 11, graph replay of 5) records the energies, snapshots
+    and final state of the fused schedule; and FP64 row sweeps follow the
+    oracle's chain stage by stage."""
+    kw = film(nx=64, ny=48, nz=1, seed=3)
+    out = []
+    for rows in (0, 1):
+        c = mmc.Context(mmc.make_params(**kw), 1)
+        c.build_tensor()
+        c.init_random_m()
+        c.set_option("rows", rows)
+        c.set_option("graph", 1)
+        c.set_option("graph_sweeps", 5)
+        E, S = [], []
+        c.anneal(1400.0, 700.0, 0.8, True, 23, energy_period=7, snapshot_period=11,
+                 on_energy=lambda T, st, sw, e: E.append((st, sw, e)),
+                 on_snapshot=lambda T, st, sw, m: S.append(m))
+        assert len(E) > 3 and len(S) > 3
+        out.append((E, S, c.m()))
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][2], out[1][2])
+    o, c = pair(orc, mmc, **kw)
+    m = random_m((64, 48, 1), 9)
+    o.set_m(m)
+    c.set_m(m)
+    c.set_option("rows", 1)
+    for st, T in enumerate((1100.0, 500.0, 90.0)):
+        c.set_temperature(T)
+        c.sweeps(st, 0, 9)
+        for k in range(9):
+            o.sweep(T, st, k)
+    assert rel_inf(c.m(), o.m()) <= 1e-10
