@@ -305,6 +305,22 @@ __device__ __forceinline__ void staged_store(int count, const Staged<(MAXC + NT 
     }
 }
 
+// Twiddle table of a compile-time length N into LDS in two steps: the loads
+// (load_twiddles) are issued before the kernel's data loads and the LDS
+// writes (store_twiddles) done after them, so the table's latency overlaps
+// the data's instead of preceding it.
+template <typename T, int N, int NT = kThreads>
+__device__ __forceinline__ Staged<(N + NT - 1) / NT, cpx<T>> load_twiddles(const cpx<T> *__restrict__ src)
+{
+    return staged_load<N, NT, cpx<T>>(N, [&](int e) { return src[e]; });
+}
+
+template <typename T, int N, int NT = kThreads>
+__device__ __forceinline__ void store_twiddles(cpx<T> *dst, const Staged<(N + NT - 1) / NT, cpx<T>> &r)
+{
+    staged_store<N, NT, cpx<T>>(N, r, [&](int e, cpx<T> v) { dst[e] = v; });
+}
+
 // Twiddle table of a compile-time length N into LDS, loads batched.
 template <typename T, int N, int NT = kThreads>
 __device__ __forceinline__ void stage_twiddles(cpx<T> *dst, const cpx<T> *__restrict__ src)
@@ -411,14 +427,11 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
             const int k = e / npair, q = e - k * npair;
             const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
             Pair2<cpx<T>> v;
+            // branch-free: p[1] read in place of a missing row, zero selected
+            const bool hb = 2 * q + 1 < nyt;
             v.a = p[0];
-            T bx = T(0), by = T(0);
-            if (2 * q + 1 < nyt) {
-                const cpx<T> b = p[1];
-                bx = b.x;
-                by = b.y;
-            }
-            v.b = {bx, by};
+            const cpx<T> b = p[hb ? 1 : 0];
+            v.b = {hb ? b.x : T(0), hb ? b.y : T(0)};
             return v;
         },
         [&](int e, Pair2<cpx<T>> v) {
@@ -456,7 +469,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd_n(const Inst *__restrict__ I,
     const T *src = realb + in.m_off + (long)c * in.N + (long)z * Lx * Ly;
     cpx<T> *dst = specb + in.a_off + (long)c * H * Lz * Ly;
     const int nyt = min(2 * ROWS, Ly - y0);
-    stage_twiddles<T, N>(tw, twb + in.twx);
+    const auto twr = load_twiddles<T, N>(twb + in.twx);
     staged_copy<ROWS * N, kThreads, cpx<T>>(
         ROWS * N,
         [&](int e) {
@@ -470,6 +483,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd_n(const Inst *__restrict__ I,
             return cpx<T>{re, im};
         },
         [&](int e, cpx<T> v) { s[e] = v; });
+    store_twiddles<T, N>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
     const int npair = (nyt + 1) >> 1;
@@ -501,7 +515,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
     T *dst = realb + in.h_off + (long)c * in.N + (long)z * Lx * Ly;
     const int nyt = min(2 * ROWS, Ly - y0);
     const int npair = (nyt + 1) >> 1;
-    stage_twiddles<T, N>(tw, twb + in.twx);
+    const auto twr = load_twiddles<T, N>(twb + in.twx);
     for (int e = threadIdx.x; e < (ROWS - npair) * N; e += kThreads) s[npair * N + e] = {T(0), T(0)};
     staged_copy<H * ROWS, kThreads, Pair2<cpx<T>>>(
         H * npair,
@@ -509,14 +523,11 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
             const int k = e / npair, q = e - k * npair;
             const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
             Pair2<cpx<T>> v;
+            // branch-free: p[1] read in place of a missing row, zero selected
+            const bool hb = 2 * q + 1 < nyt;
             v.a = p[0];
-            T bx = T(0), by = T(0);
-            if (2 * q + 1 < nyt) {
-                const cpx<T> b = p[1];
-                bx = b.x;
-                by = b.y;
-            }
-            v.b = {bx, by};
+            const cpx<T> b = p[hb ? 1 : 0];
+            v.b = {hb ? b.x : T(0), hb ? b.y : T(0)};
             return v;
         },
         [&](int e, Pair2<cpx<T>> v) {
@@ -525,6 +536,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
             s[q * N + k] = {a.x - b.y, -(a.y + b.x)};
             if (k > 0 && N - k >= H) s[q * N + N - k] = {a.x + b.y, a.y - b.x};
         });
+    store_twiddles<T, N>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
     for (int e = threadIdx.x; e < nyt * Lx; e += kThreads) {
@@ -735,7 +747,7 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
         #pragma unroll
         for (int q = 0; q < (PF ? Q : 1); ++q) load_k6(q, k6p[q]);
     }
-    stage_twiddles<T, N>(tw, twb + in.twy);
+    const auto twr = load_twiddles<T, N>(twb + in.twy);
     staged_copy<ROWS * N, kThreads, cpx<T>>(
         ROWS * N,
         [&](int e) {
@@ -751,6 +763,7 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
             return cpx<T>{vx, vy};
         },
         [&](int e, cpx<T> v) { s[e] = v; });
+    store_twiddles<T, N>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
     // without prefetch, all k points' values are loaded before the first product
@@ -856,7 +869,7 @@ void k_yzconv_n(const Inst *__restrict__ I,
                 k6[d][c6] = ky < N ? khat_at(kr, in, c6, kstride, ki) : cpx<T>{T(0), T(0)};
         }
     };
-    stage_twiddles<T, N / 4, NT>(tw, twb + in.twy);
+    const auto twr = load_twiddles<T, N / 4, NT>(twb + in.twy);
     cpx<T> kp[PF ? Q : 1][ND][6];
     auto row_ld = [&](int e) {
         const int r = e / N, y = e - r * N;
@@ -877,6 +890,7 @@ void k_yzconv_n(const Inst *__restrict__ I,
     } else {
         staged_copy<ROWS * N, NT, cpx<T>>(ROWS * N, row_ld, row_st);
     }
+    store_twiddles<T, N / 4, NT>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, ROWS, NT, true>(s, tw);
     // one k point at a time (a rolled loop keeps the next point's tensor
@@ -1463,7 +1477,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
     const int R = RZ / nzl;
     const int nyt = min(2 * R, ny - y0);
     const int npair = (nyt + 1) >> 1;
-    stage_twiddles<T, N>(tw, twb + in.twx);
+    const auto twr = load_twiddles<T, N>(twb + in.twx);
     // Colour-0 cells of this thread: e = threadIdx.x + q*kThreads < ncell
     // (at most QC).  Their m values, neighbours and trial moves are fetched
     // after the transform, CH cells at a time with the loads issued together
@@ -1520,14 +1534,11 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
                 const int k = r / npair, q = r - k * npair;
                 const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
                 Pair2<cpx<T>> v;
+                // branch-free: p[1] read in place of a missing row, zero selected
+                const bool hb = 2 * q + 1 < nyt;
                 v.a = p[0];
-                T bx = T(0), by = T(0);
-                if (2 * q + 1 < nyt) {
-                    const cpx<T> b = p[1];
-                    bx = b.x;
-                    by = b.y;
-                }
-                v.b = {bx, by};
+                const cpx<T> b = p[hb ? 1 : 0];
+                v.b = {hb ? b.x : T(0), hb ? b.y : T(0)};
                 return v;
             },
             [&](int e, Pair2<cpx<T>> v) {
@@ -1569,6 +1580,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
             }
         }
     }
+    store_twiddles<T, N>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, 3 * RZ>(s, tw);
     auto hval = [&](int zl, int yy, int x) -> V3<T> {
@@ -1639,7 +1651,7 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
     const int nyt = min(2 * R, ny - y0);
     const long NN = in.N;
     T *m = mb + in.m_off;
-    stage_twiddles<T, N>(tw, twb + in.twx);
+    const auto twr = load_twiddles<T, N>(twb + in.twx);
     // Colour-1 cells of this thread: e = threadIdx.x + q*kThreads < ncell
     // (at most QC).  Their compact h, neighbours (colour 0, not written by
     // this kernel) and trial moves are fetched CH cells at a time, loads
@@ -1698,6 +1710,7 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
     } else {
         staged_copy<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld, row_st);
     }
+    store_twiddles<T, N>(tw, twr);
     __syncthreads();
     {
         const T bt = T(beta[w.inst]);
@@ -1839,7 +1852,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         return y;
     };
     auto cell = [&](int y, int z, int x) { return (long)nx * (y + (long)ny * z) + x; };
-    stage_twiddles<T, N, NT>(tw, twb + in.twx);
+    const auto twr = load_twiddles<T, N, NT>(twb + in.twx);
     // Cells of this thread: colour 0 on rows -1..2 (QC0 at most, CH0 per
     // gather) and colour 1 on rows 0, 1 (QC1 at most), all layers.  Their
     // inputs from m_in and their trial moves depend on nothing this kernel
@@ -1900,21 +1913,13 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         const int rr = e / H, k = e - rr * H;
         const int pz = rr / 3, p = pz / NZ, z = pz - p * NZ, c = rr - 3 * pz;
         const int ya = row_of(2 * (p - 1)), yb = row_of(2 * (p - 1) + 1);
+        // branch-free: loads from clamped rows, zeros selected after
+        const bool ha = ya >= 0, hb = ha && yb >= 0;
+        const cpx<T> *col = A + c * cstride + ((long)k * NZ + z) * ny;
+        const cpx<T> a = col[ha ? ya : 0], b = col[hb ? yb : (ha ? ya : 0)];
         Pair2<cpx<T>> v;
-        T ax = T(0), ay = T(0), bx = T(0), by = T(0);
-        if (ya >= 0) {
-            const cpx<T> *col = A + c * cstride + ((long)k * NZ + z) * ny;
-            const cpx<T> a = col[ya];
-            ax = a.x;
-            ay = a.y;
-            if (yb >= 0) {
-                const cpx<T> b = col[yb];
-                bx = b.x;
-                by = b.y;
-            }
-        }
-        v.a = {ax, ay};
-        v.b = {bx, by};
+        v.a = {ha ? a.x : T(0), ha ? a.y : T(0)};
+        v.b = {hb ? b.x : T(0), hb ? b.y : T(0)};
         return v;
     };
     auto spec_st = [&](int e, Pair2<cpx<T>> v) {
@@ -1935,6 +1940,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     } else {
         staged_copy<PER, NT, Pair2<cpx<T>>>(PER, spec_ld, spec_st);
     }
+    store_twiddles<T, N, NT>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, PR, NT>(s, tw);
     // the pair's slots outside the film are zero for the forward transform
