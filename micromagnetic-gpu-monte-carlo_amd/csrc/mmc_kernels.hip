@@ -332,6 +332,33 @@ __device__ __forceinline__ void stage_twiddles(cpx<T> *dst, const cpx<T> *__rest
 // Two values loaded together by one thread of a staged copy.
 template <typename V> struct Pair2 { V a, b; };
 
+// Rows (y, y+1) of the y-contiguous spectrum at one (component, kx, z) as
+// one 16-byte access.  Lanes of an x-pass tile read along kx, a line apart
+// each, so halving the accesses halves the address work of those loads.
+// Valid for FP32 when every pair is complete (even ny) and the instance's
+// spectrum starts 16-byte aligned: pair16(in).
+template <typename T>
+__device__ __forceinline__ bool pair16(const Inst &in)
+{
+    return sizeof(T) == 4 && (in.ny & 1) == 0 && (in.a_off & 1) == 0;
+}
+
+template <typename T>
+__device__ __forceinline__ Pair2<cpx<T>> load_pair16(const cpx<T> *p)
+{
+    const float4 f = *reinterpret_cast<const float4 *>(p);
+    Pair2<cpx<T>> v;
+    v.a = {T(f.x), T(f.y)};
+    v.b = {T(f.z), T(f.w)};
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_pair16(cpx<T> *p, const cpx<T> &a, const cpx<T> &b)
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(float(a.x), float(a.y), float(b.x), float(b.y));
+}
+
 // Copy a twiddle table into LDS (after the two ping-pong buffers); callers
 // synchronise once data and twiddles are both loaded.
 template <typename T>
@@ -355,6 +382,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd(const Inst *__restrict__ I,
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int c = w.a, z = w.b, y0 = w.c;
     const int P = in.Px, H = in.Hx;
     int Lx, Ly, Lz, ty;
@@ -395,8 +423,14 @@ __global__ __launch_bounds__(kThreads) void k_xfwd(const Inst *__restrict__ I,
         const cpx<T> A = s[q * P + k];
         const cpx<T> B = s[q * P + (k ? P - k : 0)];
         cpx<T> *o = dst + ((long)k * Lz + z) * Ly + y0 + 2 * q;
-        o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
-        if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+        const cpx<T> oa = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+        const cpx<T> ob = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+        if (p16) {
+            store_pair16(o, oa, ob);
+        } else {
+            o[0] = oa;
+            if (2 * q + 1 < nyt) o[1] = ob;
+        }
     }
 }
 
@@ -410,6 +444,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int c = w.a, z = w.b, y0 = w.c;
     const int P = in.Px, H = in.Hx;
     const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
@@ -426,6 +461,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv(const Inst *__restrict__ I,
         [&](int e) {
             const int k = e / npair, q = e - k * npair;
             const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+            if (p16) return load_pair16(p);
             Pair2<cpx<T>> v;
             // branch-free: p[1] read in place of a missing row, zero selected
             const bool hb = 2 * q + 1 < nyt;
@@ -463,6 +499,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd_n(const Inst *__restrict__ I,
     cpx<T> *tw = s + ROWS * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int c = w.a, z = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
@@ -492,8 +529,14 @@ __global__ __launch_bounds__(kThreads) void k_xfwd_n(const Inst *__restrict__ I,
         const cpx<T> A = s[q * N + k];
         const cpx<T> B = s[q * N + (k ? N - k : 0)];
         cpx<T> *o = dst + ((long)k * Lz + z) * Ly + y0 + 2 * q;
-        o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
-        if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+        const cpx<T> oa = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+        const cpx<T> ob = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+        if (p16) {
+            store_pair16(o, oa, ob);
+        } else {
+            o[0] = oa;
+            if (2 * q + 1 < nyt) o[1] = ob;
+        }
     }
 }
 
@@ -508,6 +551,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
     cpx<T> *tw = s + ROWS * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int c = w.a, z = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int Lx = in.nx, Ly = in.ny, Lz = in.nz;
@@ -522,6 +566,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_n(const Inst *__restrict__ I,
         [&](int e) {
             const int k = e / npair, q = e - k * npair;
             const cpx<T> *p = src + ((long)k * Lz + z) * Ly + y0 + 2 * q;
+            if (p16) return load_pair16(p);
             Pair2<cpx<T>> v;
             // branch-free: p[1] read in place of a missing row, zero selected
             const bool hb = 2 * q + 1 < nyt;
@@ -1470,6 +1515,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
     cpx<T> *tw = s + 3 * RZ * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int z0 = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
@@ -1533,6 +1579,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
                 const int c = e / per, r = e - c * per;
                 const int k = r / npair, q = r - k * npair;
                 const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
+                if (p16) return load_pair16(p);
                 Pair2<cpx<T>> v;
                 // branch-free: p[1] read in place of a missing row, zero selected
                 const bool hb = 2 * q + 1 < nyt;
@@ -1643,6 +1690,7 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
     cpx<T> *tw = s + 3 * RZ * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int z0 = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
@@ -1762,8 +1810,14 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
             const cpx<T> A = row[k];
             const cpx<T> B = row[k ? N - k : 0];
             cpx<T> *o = dst + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
-            o[0] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
-            if (2 * q + 1 < nyt) o[1] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+            const cpx<T> oa = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+            const cpx<T> ob = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+            if (p16) {
+                store_pair16(o, oa, ob);
+            } else {
+                o[0] = oa;
+                if (2 * q + 1 < nyt) o[1] = ob;
+            }
         }
     } else {
         const int Pz = in.Pz;
@@ -1832,6 +1886,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     cpx<T> *tw = s + PR * N;
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
     const int y0 = w.c;
     const int nx = in.nx, ny = in.ny, pery = in.pbc[1];
     const int hw = (nx + 1) >> 1;
@@ -1916,6 +1971,13 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         // branch-free: loads from clamped rows, zeros selected after
         const bool ha = ya >= 0, hb = ha && yb >= 0;
         const cpx<T> *col = A + c * cstride + ((long)k * NZ + z) * ny;
+        if (p16) {   // complete pairs: yb = ya + 1 whenever ya >= 0
+            const Pair2<cpx<T>> w = load_pair16(col + (ha ? ya : 0));
+            Pair2<cpx<T>> v;
+            v.a = {ha ? w.a.x : T(0), ha ? w.a.y : T(0)};
+            v.b = {ha ? w.b.x : T(0), ha ? w.b.y : T(0)};
+            return v;
+        }
         const cpx<T> a = col[ha ? ya : 0], b = col[hb ? yb : (ha ? ya : 0)];
         Pair2<cpx<T>> v;
         v.a = {ha ? a.x : T(0), ha ? a.y : T(0)};
@@ -2051,8 +2113,14 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         const cpx<T> Z = row[k];
         const cpx<T> B = row[k ? N - k : 0];
         cpx<T> *o = A + c * cstride + ((long)k * NZ + z) * ny + y0;
-        o[0] = {(Z.x + B.x) * T(0.5), (Z.y - B.y) * T(0.5)};
-        if (nyt > 1) o[1] = {(Z.y + B.y) * T(0.5), (B.x - Z.x) * T(0.5)};
+        const cpx<T> oa = {(Z.x + B.x) * T(0.5), (Z.y - B.y) * T(0.5)};
+        const cpx<T> ob = {(Z.y + B.y) * T(0.5), (B.x - Z.x) * T(0.5)};
+        if (p16) {
+            store_pair16(o, oa, ob);
+        } else {
+            o[0] = oa;
+            if (nyt > 1) o[1] = ob;
+        }
     }
 }
 
