@@ -1962,11 +1962,21 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     };
     // spectrum of the three pairs of every layer, stored conjugated for the
     // inverse transform (as k_xinv_metro); a pair outside an open axis stays
-    // zero.  Element e = ((p*NZ + z)*3 + c)*H + k.
+    // zero.  Element e = ((k*3 + c)*NZ + z)*3 + p: the three pairs of one
+    // (kx, component, layer) are six consecutive rows, so neighbouring lanes
+    // read neighbouring 16-byte pieces of one line.
     constexpr int PER = PR * H;
+    auto decode = [&](int e, int &p, int &z, int &c, int &k) {
+        p = e % 3;
+        const int t = e / 3;
+        z = t % NZ;
+        const int t2 = t / NZ;
+        c = t2 % 3;
+        k = t2 / 3;
+    };
     auto spec_ld = [&](int e) {
-        const int rr = e / H, k = e - rr * H;
-        const int pz = rr / 3, p = pz / NZ, z = pz - p * NZ, c = rr - 3 * pz;
+        int p, z, c, k;
+        decode(e, p, z, c, k);
         const int ya = row_of(2 * (p - 1)), yb = row_of(2 * (p - 1) + 1);
         // branch-free: loads from clamped rows, zeros selected after
         const bool ha = ya >= 0, hb = ha && yb >= 0;
@@ -1985,7 +1995,9 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         return v;
     };
     auto spec_st = [&](int e, Pair2<cpx<T>> v) {
-        const int rr = e / H, k = e - rr * H;
+        int p, z, c, k;
+        decode(e, p, z, c, k);
+        const int rr = (p * NZ + z) * 3 + c;
         const cpx<T> a = v.a;
         const T bx = v.b.x, by = v.b.y;
         cpx<T> *row = s + rr * N;
