@@ -291,8 +291,12 @@ int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
         return fail(c, MMC_EINVAL, "instance %d: cell size must be positive", idx);
     if (!(p.ms > 0))
         return fail(c, MMC_EINVAL, "instance %d: ms must be positive", idx);
-    if ((long)p.nx * p.ny * p.nz > (1L << 31) - 1)
-        return fail(c, MMC_EINVAL, "instance %d: more than 2^31-1 cells", idx);
+    // kernels index the three components of m as i, N+i, 2N+i in 32 bits,
+    // with cell indices x + nx*(y + ny*z) from 24-bit products
+    if ((long)p.nx * p.ny * p.nz > ((1L << 32) - 1) / 3)
+        return fail(c, MMC_EINVAL, "instance %d: more than (2^32-1)/3 cells", idx);
+    if (p.nx >= (1 << 24) || (long)p.ny * p.nz >= (1L << 24))
+        return fail(c, MMC_EINVAL, "instance %d: nx and ny*nz must be below 2^24", idx);
     std::memset(&in, 0, sizeof(in));
     in.nx = p.nx; in.ny = p.ny; in.nz = p.nz;
     in.pbc[0] = p.pbc_x != 0; in.pbc[1] = p.pbc_y != 0; in.pbc[2] = p.pbc_z != 0;

@@ -36,12 +36,14 @@ namespace mmc {
 // ============================================================================
 // small helpers
 // ============================================================================
+// Neighbour of i at step d = +-1 along an axis of n cells: -1 past an open
+// end or when the wrap lands on i itself (n = 1).
 __device__ __forceinline__ int nbr(int i, int d, int n, int periodic)
 {
     int j = i + d;
     if (j < 0 || j >= n) {
         if (!periodic) return -1;
-        j = (j + n) % n;
+        j = j < 0 ? j + n : j - n;   // j is -1 or n: (j + n) mod n without a division
     }
     return j == i ? -1 : j;
 }
@@ -1088,10 +1090,21 @@ __global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict
 // ============================================================================
 template <typename T> struct V3 { T x, y, z; };
 
-template <typename T>
-__device__ __forceinline__ V3<T> load_m(const T *m, long N, long i)
+// Cell indices of one instance are 32-bit (the host keeps 3N below 2^32):
+// cheaper than 64-bit arithmetic, whose multiply-adds read undefined high
+// halves of registers that in-flight loads write, so the compiler waited for
+// those loads.
+// x + nx*(y + ny*z) with 24-bit products (full-rate multiply-adds; the host
+// keeps nx and ny*nz below 2^24)
+__device__ __forceinline__ uint32_t cell_index(uint32_t nx, uint32_t ny, int x, int y, int z)
 {
-    return {m[i], m[N + i], m[2 * N + i]};
+    return (uint32_t)x + __umul24(nx, (uint32_t)y + __umul24(ny, (uint32_t)z));
+}
+
+template <typename T>
+__device__ __forceinline__ V3<T> load_m(const T *m, uint32_t N, uint32_t i)
+{
+    return {m[i], m[N + i], m[2u * N + i]};
 }
 
 // Bond sums of one cell from its six neighbour values (a missing bond
@@ -1129,20 +1142,21 @@ template <typename T>
 __device__ __forceinline__ void stencil(const Inst &in, const T *m, int x, int y, int z,
                                         V3<T> &ex, V3<T> &dmi, T &cnt)
 {
-    const long N = in.N;
+    const uint32_t N = (uint32_t)in.N;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
     const V3<T> zero = {T(0), T(0), T(0)};
     V3<T> xp = zero, xm = zero, yp = zero, ym = zero, zp = zero, zm = zero;
     cnt = T(0);
-    const long row = (long)nx * (y + (long)ny * z);
+    const uint32_t unx = nx, uny = ny;
+    const uint32_t row = cell_index(unx, uny, 0, y, z);
     int j;
     if ((j = nbr(x, +1, nx, in.pbc[0])) >= 0) { xp = load_m(m, N, row + j); cnt += T(in.lam[0]); }
     if ((j = nbr(x, -1, nx, in.pbc[0])) >= 0) { xm = load_m(m, N, row + j); cnt += T(in.lam[0]); }
-    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) { yp = load_m(m, N, x + (long)nx * (j + (long)ny * z)); cnt += T(in.lam[1]); }
-    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) { ym = load_m(m, N, x + (long)nx * (j + (long)ny * z)); cnt += T(in.lam[1]); }
+    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) { yp = load_m(m, N, cell_index(unx, uny, x, j, z)); cnt += T(in.lam[1]); }
+    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) { ym = load_m(m, N, cell_index(unx, uny, x, j, z)); cnt += T(in.lam[1]); }
     if (nz > 1) {
-        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) { zp = load_m(m, N, x + (long)nx * (y + (long)ny * j)); cnt += T(in.lam[2]); }
-        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) { zm = load_m(m, N, x + (long)nx * (y + (long)ny * j)); cnt += T(in.lam[2]); }
+        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) { zp = load_m(m, N, cell_index(unx, uny, x, y, j)); cnt += T(in.lam[2]); }
+        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) { zm = load_m(m, N, cell_index(unx, uny, x, y, j)); cnt += T(in.lam[2]); }
     }
     bond_sums(in, xp, xm, yp, ym, zp, zm, ex, dmi);
 }
@@ -1245,19 +1259,20 @@ template <typename T> struct Nbrs {
 template <typename T>
 __device__ __forceinline__ Nbrs<T> gather_nbrs(const Inst &in, const T *m, int x, int y, int z)
 {
-    const long N = in.N;
+    const uint32_t N = (uint32_t)in.N;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
     const V3<T> zero = {T(0), T(0), T(0)};
     Nbrs<T> b = {zero, zero, zero, zero, zero, zero};
-    const long row = (long)nx * (y + (long)ny * z);
+    const uint32_t unx = nx, uny = ny;
+    const uint32_t row = cell_index(unx, uny, 0, y, z);
     int j;
     if ((j = nbr(x, +1, nx, in.pbc[0])) >= 0) b.xp = load_m(m, N, row + j);
     if ((j = nbr(x, -1, nx, in.pbc[0])) >= 0) b.xm = load_m(m, N, row + j);
-    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) b.yp = load_m(m, N, x + (long)nx * (j + (long)ny * z));
-    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) b.ym = load_m(m, N, x + (long)nx * (j + (long)ny * z));
+    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) b.yp = load_m(m, N, cell_index(unx, uny, x, j, z));
+    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) b.ym = load_m(m, N, cell_index(unx, uny, x, j, z));
     if (nz > 1) {
-        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) b.zp = load_m(m, N, x + (long)nx * (y + (long)ny * j));
-        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) b.zm = load_m(m, N, x + (long)nx * (y + (long)ny * j));
+        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) b.zp = load_m(m, N, cell_index(unx, uny, x, y, j));
+        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) b.zm = load_m(m, N, cell_index(unx, uny, x, y, j));
     }
     return b;
 }
@@ -1906,7 +1921,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         }
         return y;
     };
-    auto cell = [&](int y, int z, int x) { return (long)nx * (y + (long)ny * z) + x; };
+    auto cell = [&](int y, int z, int x) { return cell_index(nx, ny, x, y, z); };
     const auto twr = load_twiddles<T, N, NT>(twb + in.twx);
     // Cells of this thread: colour 0 on rows -1..2 (QC0 at most, CH0 per
     // gather) and colour 1 on rows 0, 1 (QC1 at most), all layers.  Their
@@ -1945,19 +1960,19 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         int r, z, y, x;
         cell0(q, r, z, y, x);
         if (y >= 0) {
-            const long i = cell(y, z, x);
+            const uint32_t i = cell(y, z, x);
             c0mi[k] = load_m(mi_, NN, i);
             c0nb[k] = gather_nbrs(in, mi_, x, y, z);
-            c0mv[k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
+            c0mv[k] = trial_move<T>(in, i, sweep, stage);
         }
     };
     auto fetch1 = [&](int q, int k) {
         int r, z, y, x;
         cell1(q, r, z, y, x);
         if (y >= 0) {
-            const long i = cell(y, z, x);
+            const uint32_t i = cell(y, z, x);
             c1mi[k] = load_m(mi_, NN, i);
-            c1mv[k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
+            c1mv[k] = trial_move<T>(in, i, sweep, stage);
         }
     };
     // spectrum of the three pairs of every layer, stored conjugated for the
@@ -1974,45 +1989,62 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
         c = t2 % 3;
         k = t2 / 3;
     };
-    auto spec_ld = [&](int e) {
-        int p, z, c, k;
-        decode(e, p, z, c, k);
+    // rows present of pairs p = 0, 1, 2 (bits 2p, 2p+1), the same for every
+    // thread: the loads read clamped rows unconditionally and the zeros are
+    // selected when the values are stored, so no load is waited for early
+    unsigned okm = 0;
+    #pragma unroll
+    for (int p = 0; p < 3; ++p) {
         const int ya = row_of(2 * (p - 1)), yb = row_of(2 * (p - 1) + 1);
-        // branch-free: loads from clamped rows, zeros selected after
-        const bool ha = ya >= 0, hb = ha && yb >= 0;
-        const cpx<T> *col = A + c * cstride + ((long)k * NZ + z) * ny;
-        if (p16) {   // complete pairs: yb = ya + 1 whenever ya >= 0
-            const Pair2<cpx<T>> w = load_pair16(col + (ha ? ya : 0));
-            Pair2<cpx<T>> v;
-            v.a = {ha ? w.a.x : T(0), ha ? w.a.y : T(0)};
-            v.b = {ha ? w.b.x : T(0), ha ? w.b.y : T(0)};
-            return v;
-        }
-        const cpx<T> a = col[ha ? ya : 0], b = col[hb ? yb : (ha ? ya : 0)];
-        Pair2<cpx<T>> v;
-        v.a = {ha ? a.x : T(0), ha ? a.y : T(0)};
-        v.b = {hb ? b.x : T(0), hb ? b.y : T(0)};
-        return v;
-    };
+        okm |= (ya >= 0 ? 1u : 0u) << (2 * p);
+        okm |= (ya >= 0 && yb >= 0 ? 1u : 0u) << (2 * p + 1);
+    }
     auto spec_st = [&](int e, Pair2<cpx<T>> v) {
         int p, z, c, k;
         decode(e, p, z, c, k);
+        const bool ha = (okm >> (2 * p)) & 1u, hb = (okm >> (2 * p + 1)) & 1u;
         const int rr = (p * NZ + z) * 3 + c;
-        const cpx<T> a = v.a;
-        const T bx = v.b.x, by = v.b.y;
+        const T ax = ha ? v.a.x : T(0), ay = ha ? v.a.y : T(0);
+        const T bx = hb ? v.b.x : T(0), by = hb ? v.b.y : T(0);
         cpx<T> *row = s + rr * N;
-        row[k] = {a.x - by, -(a.y + bx)};
-        if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
+        row[k] = {ax - by, -(ay + bx)};
+        if (k > 0 && N - k >= H) row[N - k] = {ax + by, ay - bx};
     };
-    if (PF) {
-        const auto spec = staged_load<PER, NT, Pair2<cpx<T>>>(PER, spec_ld);
-        #pragma unroll
-        for (int q = 0; q < QC0; ++q) fetch0(q, q);
-        #pragma unroll
-        for (int q = 0; q < QC1; ++q) fetch1(q, q);
-        staged_store<PER, NT, Pair2<cpx<T>>>(PER, spec, spec_st);
+    // the access width is fixed for the whole phase (one branch around it):
+    // a per-element choice lets the compiler merge both into narrow loads
+    auto load_phase = [&](auto p16c) {
+        constexpr bool P16 = decltype(p16c)::value;
+        auto spec_ld = [&](int e) {
+            int p, z, c, k;
+            decode(e, p, z, c, k);
+            const int ya = row_of(2 * (p - 1)), yb = row_of(2 * (p - 1) + 1);
+            const bool ha = ya >= 0, hb = ha && yb >= 0;
+            const cpx<T> *col = A + c * cstride + ((long)k * NZ + z) * ny;
+            Pair2<cpx<T>> v;
+            if constexpr (P16) {   // complete pairs: yb = ya + 1 whenever ya >= 0
+                v = load_pair16(col + (ha ? ya : 0));
+            } else {
+                v.a = col[ha ? ya : 0];
+                v.b = col[hb ? yb : (ha ? ya : 0)];
+            }
+            return v;
+        };
+        if (PF) {
+            const auto spec = staged_load<PER, NT, Pair2<cpx<T>>>(PER, spec_ld);
+            #pragma unroll
+            for (int q = 0; q < QC0; ++q) fetch0(q, q);
+            #pragma unroll
+            for (int q = 0; q < QC1; ++q) fetch1(q, q);
+            staged_store<PER, NT, Pair2<cpx<T>>>(PER, spec, spec_st);
+        } else {
+            staged_copy<PER, NT, Pair2<cpx<T>>>(PER, spec_ld, spec_st);
+        }
+    };
+    if constexpr (sizeof(T) == 4) {
+        if (p16) load_phase(std::integral_constant<bool, true>{});
+        else load_phase(std::integral_constant<bool, false>{});
     } else {
-        staged_copy<PER, NT, Pair2<cpx<T>>>(PER, spec_ld, spec_st);
+        load_phase(std::integral_constant<bool, false>{});
     }
     store_twiddles<T, N, NT>(tw, twr);
     __syncthreads();
