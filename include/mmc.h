@@ -63,7 +63,9 @@ enum { MMC_OK = 0, MMC_EINVAL = 1, MMC_EHIP = 2, MMC_EIO = 3,
 
 /* Allocate one instance on the current HIP device.
  * Replaces the cudaMalloc / cufftPlan3d set-up of src/main.cu:396-409,
- * :593-631, :688-756 and the padding rule :254-263. */
+ * :593-631, :688-756 and the padding rule :254-263.
+ * MMC_EINVAL for grids the kernels' 32-bit cell indexing cannot take:
+ * nx >= 2^24, ny*nz >= 2^24 or more than (2^32-1)/3 cells. */
 int mmc_create(const mmc_params *p, int precision, mmc_ctx **out);
 
 /* Allocate `count` independent instances (uneven sizes allowed) that are

@@ -621,3 +621,12 @@ def test_row_sweeps_in_anneal(orc, mmc):
         for k in range(9):
             o.sweep(T, st, k)
     assert rel_inf(c.m(), o.m()) <= 1e-10
+
+
+def test_grid_index_limits(mmc):
+    """Grids whose cell indices would not fit the kernels' 32-bit index
+    arithmetic (nx or ny*nz at 2^24 or more) are refused when the context is
+    created, before any device memory is allocated."""
+    for geo in (dict(nx=1 << 24, ny=1, nz=1), dict(nx=1, ny=1 << 12, nz=1 << 12)):
+        with pytest.raises(mmc.MMCError, match="2\\^24"):
+            mmc.Context(mmc.make_params(**film(**geo)), 1)
