@@ -1927,10 +1927,12 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     // gather) and colour 1 on rows 0, 1 (QC1 at most), all layers.  Their
     // inputs from m_in and their trial moves depend on nothing this kernel
     // computes, so for short rows (PF) they are all fetched before the
-    // transform, behind the spectrum loads.
+    // transform, behind the spectrum loads: FP32 up to two colour-0 and one
+    // colour-1 cell per thread; FP64, whose values take twice the registers,
+    // single layers up to 384 points (one cell of each colour).
     constexpr int QC0 = (4 * NZ * (N / 2) + NT - 1) / NT;
     constexpr int QC1 = (2 * NZ * (N / 2) + NT - 1) / NT;
-    constexpr bool PF = sizeof(T) == 4 && QC0 <= 2 && QC1 <= 1;
+    constexpr bool PF = sizeof(T) == 4 ? QC0 <= 2 && QC1 <= 1 : NZ == 1 && N <= 384;
     constexpr int CH0 = PF ? QC0 : (QC0 < gather_chunk<T>() ? QC0 : gather_chunk<T>());
     constexpr int CH1 = PF ? QC1 : (QC1 < gather_chunk<T>() ? QC1 : gather_chunk<T>());
     const int ncell0 = 4 * NZ * hw, ncell1 = 2 * NZ * hw;
