@@ -137,9 +137,25 @@ __device__ __forceinline__ void fft_stage(const cpx<T> *__restrict__ in,
     // instead of integer division
     const bool p2 = ((nb & (nb - 1)) == 0) && ((Ns & (Ns - 1)) == 0);
     const int lnb = __builtin_ctz((unsigned)nb);
+    // Strided elements (es > 1, rows side by side: the z pass's y columns):
+    // neighbouring lanes take neighbouring rows, so their LDS accesses are
+    // adjacent; with butterflies along a row on neighbouring lanes they would
+    // be es apart, all in one bank.  Same butterflies, other lanes.
+    const bool across = es != 1;
+    const bool p2r = (nrows & (nrows - 1)) == 0;
+    const int lnr = __builtin_ctz((unsigned)nrows);
     for (int t = threadIdx.x; t < total; t += kThreads) {
         int row, j, k;
-        if (p2) {
+        if (across) {
+            if (p2r) {
+                row = t & (nrows - 1);
+                j = t >> lnr;
+            } else {
+                j = t / nrows;
+                row = t - j * nrows;
+            }
+            k = p2 ? (j & (Ns - 1)) : j % Ns;
+        } else if (p2) {
             row = t >> lnb;
             j = t & (nb - 1);
             k = j & (Ns - 1);
