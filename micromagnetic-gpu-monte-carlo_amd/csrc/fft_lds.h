@@ -185,7 +185,7 @@ __device__ __forceinline__ void fft_stage(const cpx<T> *__restrict__ in,
 // size and layout).  Returns the buffer holding the result.  The caller must
 // have synchronised after writing s0; the result is synchronised on return.
 template <typename T>
-__device__ cpx<T> *fft_rows(cpx<T> *s0, cpx<T> *s1, int nrows, int es, int rs,
+__device__ __forceinline__ cpx<T> *fft_rows(cpx<T> *s0, cpx<T> *s1, int nrows, int es, int rs,
                             const FftPlan &pl, const cpx<T> *__restrict__ tw)
 {
     const int n = pl.n;
