@@ -133,8 +133,11 @@ int mmc_field(mmc_ctx *ctx, int inst, int terms, double *hx, double *hy,
               double *hz);
 
 /* Total energy [J], per-term energies [J] (exch, dmi, anis, zeeman, demag)
- * and the mean magnetisation, with compensated FP64 reductions.
- * Replaces calEnergyKernel + thrust::reduce (src/main.cu:902-956). */
+ * and the mean magnetisation, with compensated FP64 reductions.  One call
+ * evaluates every instance of the context; the sums are kept until the
+ * state or the tensor changes, so reading all instances of a batch costs
+ * one pass.  Replaces calEnergyKernel + thrust::reduce
+ * (src/main.cu:902-956). */
 int mmc_energy(mmc_ctx *ctx, int inst, double *E_total, double e_terms[5],
                double m_avg[3]);
 
@@ -247,7 +250,11 @@ int mmc_anneal(mmc_ctx *ctx, const mmc_schedule *s, mmc_energy_cb ecb,
  * default 0),
  * "zfold" (1: fused tiles of instances with a short padded z axis, Pz <= 8,
  * that still take z transforms span all layers and do the z transform in
- * registers, so sweeps skip the separate z passes; default 0).
+ * registers, so sweeps skip the separate z passes; default 0),
+ * "tensor_evals" (Newell evaluations per launch of the tensor build, which
+ * splits the x images into slices of at least one image; partial sums stay
+ * unnormalised between slices, so any slicing gives the bit-identical
+ * tensor; default 2e9).
  * Films of 2-4 layers (Pz <= 8) with a y length of 64, 96, 128, 192, 256,
  * 384, 512, 768, 1024 or 2048 after padding take no z transforms at all:
  * one y pass per x frequency sums the z convolution directly ("yzconv").

@@ -1,0 +1,233 @@
+// k_fused_fwd.hip -- second checkerboard colour of the Metropolis update
+// fused with the forward x pass of the updated rows.
+#include "mmc_device.h"
+
+namespace mmc {
+
+// Second checkerboard colour of a tile of real rows fused with the forward x
+// pass of the rows' final state (items and LDS rows as k_xinv_metro).  The
+// rows are loaded into LDS packed two real rows per complex row, colour-1
+// cells are updated from the compact h and written to global m and to LDS,
+// then the rows are transformed and their half spectrum stored as k_xfwd_n
+// does (zf = 0), or transformed along z in registers and stored to the
+// xz-spectrum B as k_zfwd does (zf = 1).
+template <typename T, int N, int RZ>
+__global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict__ I,
+                                                         const Work *__restrict__ W,
+                                                         T *__restrict__ mb,
+                                                         const T *__restrict__ hb,
+                                                         cpx<T> *__restrict__ specb,
+                                                         const double *__restrict__ beta,
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint32_t sweep_off,
+                                                         const cpx<T> *__restrict__ twb)
+{
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + 3 * RZ * N;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const bool p16 = pair16<T>(in);
+    const int z0 = w.b, y0 = w.c;
+    constexpr int H = N / 2 + 1;
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    const int nzl = in.zf ? nz : 1;
+    const int R = RZ / nzl;
+    const int nyt = min(2 * R, ny - y0);
+    const long NN = in.N;
+    T *m = mb + in.m_off;
+    const auto twr = load_twiddles<T, N>(twb + in.twx);
+    // Colour-1 cells of this thread: e = threadIdx.x + q*kThreads < ncell
+    // (at most QC).  Their compact h, neighbours (colour 0, not written by
+    // this kernel) and trial moves are fetched CH cells at a time, loads
+    // issued together; when they fit (PFC = QC) they are fetched together
+    // with the tile's rows.
+    const T *hc = hb + in.h_off;
+    const int hw = (nx + 1) >> 1;
+    const long cst = (long)hw * ny * nz;
+    const uint32_t sweep = ctl[1] + sweep_off, stage = ctl[0];
+    const int plane = nyt * hw;
+    const int ncell = nzl * plane;
+    constexpr int QC = (RZ * N + kThreads - 1) / kThreads;
+    constexpr int PFC = prefetch_cells<T, QC>();
+    constexpr int CH = PFC ? PFC : (QC < gather_chunk<T>() ? QC : gather_chunk<T>());
+    auto cell_at = [&](int e, int &zl, int &yy, int &j) {
+        zl = e / plane;
+        const int r = e - zl * plane;
+        yy = r / hw;
+        j = r - yy * hw;
+    };
+    V3<T> chi[CH];
+    Nbrs<T> cnb[CH];
+    Move<T> cmv[CH];
+    auto fetch = [&](int q, int k) {
+        const int e = threadIdx.x + q * kThreads;
+        int zl, yy, j;
+        cell_at(e, zl, yy, j);
+        const int z = z0 + zl, y = y0 + yy;
+        const int x = 2 * j + ((y + z + 1) & 1);
+        if (e < ncell && x < nx) {
+            const long oh = ((long)z * ny + y) * hw + j;
+            chi[k] = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
+            cnb[k] = gather_nbrs(in, m, x, y, z);
+            cmv[k] = trial_move<T>(in, (uint32_t)((long)nx * (y + (long)ny * z) + x), sweep,
+                                   stage);
+        }
+    };
+    auto row_ld = [&](int e) {
+        const int c = e / (RZ * N), r = e - c * (RZ * N);
+        const int zr = r / N, x = r - zr * N;
+        const int zl = zr / R, q = zr - zl * R;
+        T re = T(0), im = T(0);
+        if (x < nx && 2 * q < nyt) {
+            const T *row = m + c * NN + (long)nx * ((y0 + 2 * q) + (long)ny * (z0 + zl));
+            re = row[x];
+            if (2 * q + 1 < nyt) im = row[(long)nx + x];
+        }
+        return cpx<T>{re, im};
+    };
+    auto row_st = [&](int e, cpx<T> v) { s[e] = v; };
+    if (PFC) {
+        const auto rows = staged_load<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld);
+        #pragma unroll
+        for (int q = 0; q < PFC; ++q) fetch(q, q);
+        staged_store<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, rows, row_st);
+    } else {
+        staged_copy<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld, row_st);
+    }
+    store_twiddles<T, N>(tw, twr);
+    __syncthreads();
+    {
+        const T bt = T(beta[w.inst]);
+        #pragma unroll
+        for (int q0 = 0; q0 < QC; q0 += CH) {
+            if (!PFC) {
+                #pragma unroll
+                for (int k = 0; k < CH; ++k)
+                    if (q0 + k < QC) fetch(q0 + k, k);
+            }
+            #pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                const int e = threadIdx.x + (q0 + k) * kThreads;
+                if (q0 + k >= QC || e >= ncell) break;
+                int zl, yy, j;
+                cell_at(e, zl, yy, j);
+                const int z = z0 + zl, y = y0 + yy;
+                const int x = 2 * j + ((y + z + 1) & 1);
+                if (x >= nx) continue;
+                const int o = (zl * R + (yy >> 1)) * N + x;
+                const bool odd = yy & 1;
+                const V3<T> mi = odd ? V3<T>{s[o].y, s[RZ * N + o].y, s[2 * RZ * N + o].y}
+                                     : V3<T>{s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
+                const long i = (long)nx * (y + (long)ny * z) + x;
+                const V3<T> mn = metro_apply(in, m, i, mi, cnb[k], chi[k], cmv[k], bt);
+                if (odd) {
+                    s[o].y = mn.x;
+                    s[RZ * N + o].y = mn.y;
+                    s[2 * RZ * N + o].y = mn.z;
+                } else {
+                    s[o].x = mn.x;
+                    s[RZ * N + o].x = mn.y;
+                    s[2 * RZ * N + o].x = mn.z;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    fft_fixed<T, N, 3 * RZ>(s, tw);
+    const int npair = (nyt + 1) >> 1;
+    const int per = H * npair;
+    if (!in.zf) {
+        const long cstride = (long)H * nz * ny;
+        cpx<T> *dst = specb + in.a_off;
+        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+            const int c = e / per, r = e - c * per;
+            const int k = r / npair, q = r - k * npair;
+            const cpx<T> *row = s + (c * RZ + q) * N;
+            const cpx<T> A = row[k];
+            const cpx<T> B = row[k ? N - k : 0];
+            cpx<T> *o = dst + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
+            const cpx<T> oa = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+            const cpx<T> ob = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+            if (p16) {
+                store_pair16(o, oa, ob);
+            } else {
+                o[0] = oa;
+                if (2 * q + 1 < nyt) o[1] = ob;
+            }
+        }
+    } else {
+        const int Pz = in.Pz;
+        const long cstride = (long)H * Pz * ny;
+        cpx<T> *dst = specb + in.b_off;
+        const cpx<T> *twz = twb + in.twz;
+        for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
+            const int c = e / per, r = e - c * per;
+            const int k = r / npair, q = r - k * npair;
+            const bool hasb = 2 * q + 1 < nyt;
+            cpx<T> xa[kZDirect], xb[kZDirect], oa[kZDirect], ob[kZDirect];
+            #pragma unroll
+            for (int zl = 0; zl < kZDirect; ++zl) {
+                xa[zl] = xb[zl] = {T(0), T(0)};
+                if (zl < nz) {
+                    const cpx<T> *row = s + (c * RZ + zl * R + q) * N;
+                    const cpx<T> A = row[k];
+                    const cpx<T> B = row[k ? N - k : 0];
+                    xa[zl] = {(A.x + B.x) * T(0.5), (A.y - B.y) * T(0.5)};
+                    xb[zl] = {(A.y + B.y) * T(0.5), (B.x - A.x) * T(0.5)};
+                }
+            }
+            zdft_regs<T, false>(xa, oa, nz, Pz, Pz, twz);
+            zdft_regs<T, false>(xb, ob, nz, Pz, Pz, twz);
+            cpx<T> *o = dst + c * cstride + (long)k * Pz * ny + y0 + 2 * q;
+            #pragma unroll
+            for (int kz = 0; kz < kZDirect; ++kz) {
+                if (kz < Pz) {
+                    o[(long)kz * ny] = oa[kz];
+                    if (hasb) o[(long)kz * ny + 1] = ob[kz];
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
+                             int rows, T *m, const T *h, cpx<T> *spec, const double *beta,
+                             const uint32_t *ctl, uint32_t sweep_off, const cpx<T> *tw, size_t lds)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 g(n), b(kThreads);
+    constexpr int F = fused_rows_base<T>();
+#define MMC_MX_CASE(TT, NN, RR)                                                                  \
+    if (fixed_n == NN && rows == RR) {                                                           \
+        MMC_LAUNCH((k_metro_xfwd<TT, NN, RR>), g, b, lds, st, I, W, m, h, spec, beta, ctl,       \
+                   sweep_off, tw);                                                               \
+        return post();                                                                           \
+    }
+    MMC_FUSED(MMC_MX_CASE, T, F)
+#undef MMC_MX_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t set_lds_fused_fwd(int bytes)
+{
+    hipError_t e = hipSuccess;
+    constexpr int F = fused_rows_base<T>();
+#define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_metro_xfwd<TT, NN, RR>, bytes);
+    MMC_FUSED(MMC_FL, T, F)
+#undef MMC_FL
+    return e;
+}
+
+#define MMC_INST(T)                                                                           \
+    template hipError_t launch_metro_xfwd<T>(hipStream_t, const Inst *, const Work *, int, int, \
+                                             int, T *, const T *, cpx<T> *, const double *,  \
+                                             const uint32_t *, uint32_t, const cpx<T> *,     \
+                                             size_t);                                        \
+    template hipError_t set_lds_fused_fwd<T>(int);
+MMC_INST(float)
+MMC_INST(double)
+#undef MMC_INST
+
+}  // namespace mmc

@@ -147,6 +147,12 @@ struct mmc_ctx {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_sweeps = 0;
+    double tensor_evals = 2e9;        // Newell evaluations per tensor-build launch
+
+    // energy sums of the current state (8 per instance), valid until the
+    // state or the tensor changes
+    std::vector<double> e_sums;
+    bool e_valid = false;
 
     struct Rec { int kind; hipEvent_t a, b; double bytes; };
     std::vector<Rec> recs;
@@ -1102,6 +1108,7 @@ template <typename T>
 int run_sweeps(mmc_ctx *c, uint32_t stage, uint32_t sweep0, uint32_t count)
 {
     hipStream_t st = c->stream;
+    if (count) c->e_valid = false;
     if (c->any_demag && !c->tensor_ready)
         return fail(c, MMC_ESTATE, "demag tensor not built (call mmc_build_tensor)");
     if (count && fused_sweeps(c)) {
@@ -1313,6 +1320,7 @@ int refresh_mirror(mmc_ctx *c)
 
 int finish_tensor(mmc_ctx *c)
 {
+    c->e_valid = false;
     int rc = refresh_mirror(c);
     if (rc) return rc;
     rc = c->tsize == 4 ? tensor_fft<float>(c) : tensor_fft<double>(c);
@@ -1331,6 +1339,10 @@ int check_inst(const mmc_ctx *c, int inst)
 template <typename T>
 int energy_impl(mmc_ctx *c, std::vector<double> &sums)
 {
+    if (c->e_valid) {
+        sums = c->e_sums;
+        return MMC_OK;
+    }
     int rc = demag_chain<T>(c);
     if (rc) return rc;
     hipStream_t st = c->stream;
@@ -1343,6 +1355,8 @@ int energy_impl(mmc_ctx *c, std::vector<double> &sums)
     HIPCHK(c, hipMemcpyAsync(sums.data(), c->d_eout, sizeof(double) * 8 * c->count,
                              hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
+    c->e_sums = sums;
+    c->e_valid = true;
     return MMC_OK;
 }
 
@@ -1502,6 +1516,7 @@ int autotune_impl(mmc_ctx *c, int sweeps, std::string &report)
     e = hipMemcpyAsync(c->d_m, save, mbytes, hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(save);
+    c->e_valid = false;
     c->timing = timing;
     if (!rc && e != hipSuccess) rc = fail(c, MMC_EHIP, "autotune restore: %s", hipGetErrorString(e));
     return rc;
@@ -1580,7 +1595,8 @@ int mmc_build_tensor(mmc_ctx *c, const char *cache_dir)
             }
         }
     }
-    // image slices: bound each launch to ~2e9 Newell evaluations
+    // image slices: bound each launch to about c->tensor_evals Newell
+    // evaluations (a launch covers at least one x image)
     int max_img = 0;
     for (auto &in : c->inst) if (in.cal_demag) max_img = std::max(max_img, in.img[0]);
     double evals = 0;
@@ -1594,7 +1610,7 @@ int mmc_build_tensor(mmc_ctx *c, const char *cache_dir)
         Work *dW = nullptr;
         HIPCHK(c, hipMalloc(&dW, sizeof(Work) * todo.size()));
         HIPCHK(c, hipMemcpy(dW, todo.data(), sizeof(Work) * todo.size(), hipMemcpyHostToDevice));
-        const int per = std::max(1, (int)(2e9 / std::max(1.0, evals)));
+        const int per = (int)std::max(1.0, std::min(1e9, c->tensor_evals / std::max(1.0, evals)));
         int rc = MMC_OK;
         for (int lo = -max_img; lo <= max_img && rc == MMC_OK; lo += per) {
             const int hi = std::min(max_img, lo + per - 1);
@@ -1659,6 +1675,7 @@ int mmc_set_m(mmc_ctx *c, int inst, const double *mx, const double *my, const do
     if (rc) return rc;
     const Inst &in = c->inst[inst];
     const double *src[3] = {mx, my, mz};
+    c->e_valid = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < 3; ++k) {
         if (c->tsize == 8) {
@@ -1701,6 +1718,7 @@ int mmc_init_random_m(mmc_ctx *c, int inst)
     if (inst >= c->count || inst < -1) return fail(c, MMC_EINVAL, "instance %d out of range", inst);
     const int b0 = inst < 0 ? 0 : c->cl.first[inst];
     const int b1 = inst < 0 ? c->cl.n() : c->cl.first[inst + 1];
+    c->e_valid = false;
     hipError_t e = c->tsize == 4
                        ? launch_init<float>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0, (float *)c->d_m)
                        : launch_init<double>(c->stream, c->dInst, c->cl.dev + b0, b1 - b0, (double *)c->d_m);
@@ -2007,6 +2025,13 @@ int mmc_set_option(mmc_ctx *c, const char *key, long value)
     if (k == "tyx" || k == "ncol") {
         if (value < 1 || value > 4096) return fail(c, MMC_EINVAL, "%s must be in [1,4096]", key);
         return k == "tyx" ? retile(c, (int)value, 0) : retile(c, 0, (int)value);
+    }
+    if (k == "tensor_evals") {
+        // Newell evaluations per tensor-build launch; the x images are split
+        // into slices of at least one image each (1: one launch per image)
+        if (value < 1) return fail(c, MMC_EINVAL, "tensor_evals must be >= 1");
+        c->tensor_evals = (double)value;
+        return MMC_OK;
     }
     if (k == "timing") {
         if (!value) harvest(c);

@@ -45,3 +45,38 @@ def pair(orc, mmc, precision=0, shared_tensor=True, **kw):
         else:
             c.build_tensor()
     return o, c
+
+
+def newell_noise(kw, P):
+    """Rounding-noise scale of each entry of the FP64 Newell tensor, shape
+    (Pz, Py, Px): every entry sums 64 corner terms per image, each of
+    magnitude up to ~|r|^3 (|r| the image offset plus one cell diagonal, in
+    m^3), and the sum cancels to ~V/|r|^3, so two libraries whose f/g differ
+    by an ulp or two give entries that differ by about
+    eps * sqrt(64 * sum_images |r|^6) / (4 pi V).  Offsets follow the
+    tensor's slot map (open axes: o < n at o, o > -n at P + o; periodic
+    axes: the reference's centred map, images -p..p)."""
+    n = (kw["nx"], kw["ny"], kw["nz"])
+    d = (kw["dx"], kw["dy"], kw["dz"])
+    pbc = (kw.get("pbc_x", 0), kw.get("pbc_y", 0), kw.get("pbc_z", 0))
+    axes = []
+    for a in range(3):
+        i = np.arange(P[a])
+        if pbc[a]:
+            off = (i + (P[a] + 1) // 2 - 1) % P[a] - (P[a] + 1) // 2 + 1
+            imgs = np.arange(-max(pbc[a], 0), max(pbc[a], 0) + 1)
+            r = (off[:, None] + imgs[None, :] * P[a]) * d[a]       # (P, images)
+        else:
+            off = np.where(i < n[a], i, i - P[a])
+            r = (off * d[a])[:, None]
+        axes.append(r)
+    rx, ry, rz = axes
+    diag = float(np.sqrt(sum(v * v for v in d)))
+    acc = np.zeros((P[2], P[1], P[0]))
+    for kz in range(rz.shape[1]):
+        for ky in range(ry.shape[1]):
+            r2 = (rz[:, kz][:, None, None, None] ** 2 + ry[:, ky][None, :, None, None] ** 2
+                  + rx[None, None, :, :] ** 2)                     # (Pz, Py, Px, ix)
+            acc += ((np.sqrt(r2) + diag) ** 6).sum(axis=-1)
+    V = d[0] * d[1] * d[2]
+    return np.finfo(float).eps * np.sqrt(64.0 * acc) / (4 * np.pi * V)

@@ -1,0 +1,300 @@
+"""GPU parity at the anneal workloads' own sizes, against the FP64 oracle.
+
+Each workload runs the kernel variants its geometry selects in the library
+(transform lengths, tile shapes, z handling), so each is compared with the
+oracle at full size: the 256x256 film with 20 periodic images in FP64 under
+row sweeps, the 128x128x16 film (padded 256x256x32, z on the runtime-plan
+transform), one instance of every size of the batched workload, and the
+1024x1024x2 film with 10 images.  FP64: fields, energies and every cell's
+trial dE within 1e-12, three sweeps equal to the oracle's chain within 1e-10.
+FP32 bulk: H_eff and dE within 2e-6 of their largest values, energies within
+1e-6 of the sum of the term magnitudes.  Where the oracle cannot build the
+periodic tensor in test time, the GPU tensor is uploaded to it (the tensor
+build itself is compared with the oracle's on smaller periodic grids and
+in slices below).
+"""
+import numpy as np
+import pytest
+
+from helpers import film, FILM3D, newell_noise, random_m, rel_inf
+
+pytestmark = pytest.mark.gpu
+
+SCALE_KEYS = ("exch", "dmi", "anis", "zeeman", "demag")
+
+
+def _check_state(o, c, tol_h, tol_e, tol_de, inst=0, stage=3, sweep=11):
+    """H_eff, energy terms and trial dE of the library against the oracle on
+    the state both hold."""
+    err = rel_inf(c.field(31, inst), o.field(31))
+    assert err <= tol_h, f"H_eff rel err {err:.3e}"
+    Eo, Eg = o.energy(), c.energy(inst)
+    scale = sum(abs(Eo[k]) for k in SCALE_KEYS)
+    for k in ("total",) + SCALE_KEYS:
+        assert abs(Eg[k] - Eo[k]) <= tol_e * scale, f"{k}: {Eg[k]} vs {Eo[k]}"
+    assert np.abs(Eg["m_avg"] - Eo["m_avg"]).max() <= (1e-14 if tol_e < 1e-9 else 1e-7)
+    err = rel_inf(c.trial_delta_e(stage, sweep, inst), o.trial_delta_e(stage, sweep))
+    assert err <= tol_de, f"dE rel err {err:.3e}"
+
+
+def _fp64_chain(c, temps, oracles=(), stage=2):
+    """Sweeps of the library (all instances) and of the oracles at the same
+    temperatures and Philox counters."""
+    for s, T in enumerate(temps):
+        c.set_temperature(T)
+        c.sweeps(stage, s, 1)
+        for oo in oracles:
+            oo.sweep(T, stage, s)
+
+
+def test_film256_pbc20_fp64_row_sweeps(orc, mmc):
+    """256x256 film, 20 periodic images per axis, plain FP64 under the row
+    schedule (whole-sweep row-pair tiles) and under the fused schedule."""
+    kw = film(nx=256, ny=256, nz=1, pbc_x=20, pbc_y=20)
+    c = mmc.Context(mmc.make_params(**kw), 0)
+    c.build_tensor()
+    o = orc.Oracle(**kw)
+    o.set_tensor(c.tensor())
+    m = random_m((256, 256, 1), 41)
+    o.set_m(m)
+    c.set_m(m)
+    _check_state(o, c, 1e-12, 1e-12, 1e-12)
+    c.set_schedule("row sweeps, launch-sized tiles")
+    _fp64_chain(c, (1400.0, 500.0, 120.0), [o])
+    assert rel_inf(c.m(), o.m()) <= 1e-10
+    c2 = mmc.Context(mmc.make_params(**kw), 0)
+    c2.set_tensor(c.tensor())
+    c2.set_m(m)
+    c2.set_schedule("fused, minimal tiles")
+    _fp64_chain(c2, (1400.0, 500.0, 120.0))
+    assert np.array_equal(c2.m(), c.m())
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_film128x16_open(orc, mmc, precision):
+    """128x128x16 open film (padded 256x256x32): the z transforms of length 32
+    run on the runtime-plan LDS path.  The GPU tensor is also checked against
+    the oracle's build, entry by entry within the Newell rounding noise."""
+    kw = dict(FILM3D, nx=128, ny=128, nz=16, seed=128)
+    c = mmc.Context(mmc.make_params(**kw), precision)
+    assert c.P() == (256, 256, 32)
+    c.build_tensor()
+    o = orc.Oracle(**kw)
+    if precision == 0:
+        o.build_tensor()
+        _check_tensor(kw, c.tensor(), o.tensor())
+        assert np.allclose(c.mirror(), o.mirror(), rtol=0, atol=1e-12)
+    o.set_tensor(c.tensor())
+    m = random_m((128, 128, 16), 43)
+    o.set_m(m)
+    c.set_m(m)
+    if precision == 0:
+        _check_state(o, c, 1e-12, 1e-12, 1e-12)
+        _fp64_chain(c, (1400.0, 500.0, 120.0), [o])
+        assert rel_inf(c.m(), o.m()) <= 1e-10
+    else:
+        _check_state(o, c, 2e-6, 1e-6, 2e-6)
+
+
+def _batch_kws():
+    """One instance of each size of the batched workload, distinct Bz, Dind
+    and seeds."""
+    sizes = (64, 96, 128, 192, 256)
+    return [film(nx=n, ny=n, nz=1, bext_z=0.1 * (k + 1), dind=(3.0 + 0.7 * k) * 1e-3,
+                 seed=500 + k) for k, n in enumerate(sizes)]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_batch_one_instance_per_size(orc, mmc, precision):
+    """A batch holding one film of each size {64,96,128,192,256}^2 (padded x
+    and y lengths 128..512, one kernel variant per length): every instance
+    against its own oracle, and the batch's chain against the oracles'
+    chains (FP64) under the schedules a batch can take."""
+    kws = _batch_kws()
+    os_ = []
+    for kw in kws:
+        o = orc.Oracle(**kw)
+        o.build_tensor()
+        os_.append(o)
+    c = mmc.Context([mmc.make_params(**kw) for kw in kws], precision)
+    for i, o in enumerate(os_):
+        c.set_tensor(o.tensor(), i)
+    ms = [random_m((kw["nx"], kw["ny"], 1), 60 + i) for i, kw in enumerate(kws)]
+    for i, (o, m) in enumerate(zip(os_, ms)):
+        o.set_m(m)
+        c.set_m(m, i)
+    tol = (1e-12, 1e-12, 1e-12) if precision == 0 else (2e-6, 1e-6, 2e-6)
+    for i, o in enumerate(os_):
+        _check_state(o, c, *tol, inst=i)
+    if precision == 1:
+        return
+    temps = (1400.0, 500.0, 120.0)
+    _fp64_chain(c, temps, os_)
+    for i, o in enumerate(os_):
+        assert rel_inf(c.m(i), o.m()) <= 1e-10, f"instance {i}"
+    ref = [c.m(i) for i in range(len(kws))]
+    names = [ln.rsplit(None, 2)[0] for ln in c.autotune(2).splitlines()
+             if ln.endswith("us/sweep")]
+    assert len(names) >= 4
+    for name in names:
+        b = mmc.Context([mmc.make_params(**kw) for kw in kws], 0)
+        for i, o in enumerate(os_):
+            b.set_tensor(c.tensor(i), i)
+            b.set_m(ms[i], i)
+        b.set_schedule(name)
+        _fp64_chain(b, temps)
+        for i in range(len(kws)):
+            assert np.array_equal(b.m(i), ref[i]), f"{name}: instance {i}"
+        b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_film1024x2_pbc10(orc, mmc, precision):
+    """1024x1024x2 film, 10 periodic images per in-plane axis (the
+    HBM-resident workload): the direct z convolution at y length 1024 and
+    the fused x-pass + Metropolis kernels at x length 1024."""
+    kw = film(nx=1024, ny=1024, nz=2, dx=2e-9, dy=2e-9, dz=0.8e-9, pbc_x=10, pbc_y=10,
+              ms=1.1e6, aex=15e-12, dind=4e-3, ku1=1e6, bext_z=0.25, seed=1024)
+    c = mmc.Context(mmc.make_params(**kw), precision)
+    assert c.P() == (1024, 1024, 3)
+    c.build_tensor()
+    assert c.kernel_bytes("yzconv") > 0
+    o = orc.Oracle(**kw)
+    o.set_tensor(c.tensor())
+    m = random_m((1024, 1024, 2), 47)
+    o.set_m(m)
+    c.set_m(m)
+    if precision == 0:
+        _check_state(o, c, 1e-12, 1e-12, 1e-12)
+        c.set_schedule("fused, minimal tiles")
+        _fp64_chain(c, (1400.0, 500.0, 120.0), [o])
+        assert rel_inf(c.m(), o.m()) <= 1e-10
+    else:
+        _check_state(o, c, 2e-6, 1e-6, 2e-6)
+
+
+# --------------------------------------------------------------------------
+# tensor build: image slices and a 20-image periodic grid
+# --------------------------------------------------------------------------
+def test_tensor_slices_are_bit_identical(orc, mmc):
+    """The tensor build split into one launch per x image (partial sums kept
+    unnormalised between launches) equals the one-launch build bit for bit
+    and the oracle's within 1e-9."""
+    for geo in (dict(nx=16, ny=16, nz=1, pbc_x=3, pbc_y=3),
+                dict(nx=6, ny=5, nz=3, pbc_x=2, pbc_z=1)):
+        kw = film(**geo)
+        one = mmc.Context(mmc.make_params(**kw), 0)
+        one.build_tensor()
+        sl = mmc.Context(mmc.make_params(**kw), 0)
+        sl.set_option("tensor_evals", 1)
+        sl.set_option("timing", 1)
+        sl.build_tensor()
+        n, _, _ = sl.kernel_stats("tensor")
+        assert n == 2 * geo["pbc_x"] + 1
+        assert np.array_equal(sl.tensor(), one.tensor())
+        o = orc.Oracle(**kw)
+        o.build_tensor()
+        assert rel_inf(sl.tensor(), o.tensor()) <= 1e-9
+
+
+def _check_tensor(kw, Kg, Ko, margin=16.0):
+    """GPU and oracle tensors agree entry by entry within `margin` times the
+    Newell rounding-noise scale of the entry (device and host math
+    libraries round the same f/g terms differently, and each entry cancels
+    terms of size |r|^3 down to V/|r|^3).  A wrong sign, index or image of
+    any term near the entry's offset moves it by far more than that noise."""
+    noise = newell_noise(kw, Kg.shape[:0:-1])
+    scale = np.abs(Ko).max()
+    diff = np.abs(Kg - Ko)
+    ratio = (diff / (noise + 1e-12 * scale)).max()
+    print(f"tensor: max |Kg-Ko| {diff.max():.3e}, noise scale {noise.min():.2e}.."
+          f"{noise.max():.2e}, max ratio {ratio:.2f}")
+    assert (diff <= margin * noise + 1e-12 * scale).all()
+
+
+def test_tensor_pbc20_against_oracle(orc, mmc):
+    """64x64 film with 20 images per axis (41^2 image sums, several launches
+    when the evaluations per launch are capped): the GPU tensor equals the
+    oracle's within the Newell rounding noise of the image sums.  The far
+    images contribute less than that noise, so their presence is pinned by
+    the sliced build being bit-identical to the one-launch build and by the
+    3-image grids above, which agree within 1e-9."""
+    kw = film(nx=64, ny=64, nz=1, pbc_x=20, pbc_y=20)
+    c = mmc.Context(mmc.make_params(**kw), 0)
+    c.set_option("tensor_evals", 20000000)
+    c.set_option("timing", 1)
+    c.build_tensor()
+    n, _, _ = c.kernel_stats("tensor")
+    assert n > 1
+    one = mmc.Context(mmc.make_params(**kw), 0)
+    one.build_tensor()
+    assert np.array_equal(c.tensor(), one.tensor())
+    o = orc.Oracle(**kw)
+    o.build_tensor()
+    _check_tensor(kw, c.tensor(), o.tensor())
+    noise = newell_noise(kw, (64, 64, 1))[0, 0, 0]
+    assert np.allclose(c.mirror(), o.mirror(), rtol=0, atol=16 * noise)
+
+
+# --------------------------------------------------------------------------
+# strips that reach every compile-time transform length
+# --------------------------------------------------------------------------
+STRIPS = [
+    ("x384", dict(nx=192, ny=6, nz=1)),                  # Px = 384
+    ("x1024", dict(nx=512, ny=4, nz=1)),                 # Px = 1024
+    ("x2048", dict(nx=1024, ny=2, nz=1)),                # Px = 2048
+    ("x1024_pbc_2layers", dict(nx=1024, ny=4, nz=2, pbc_x=1)),
+    ("y384", dict(nx=6, ny=192, nz=1)),                  # Py = 384
+    ("y512", dict(nx=6, ny=256, nz=1)),                  # Py = 512
+    ("y768", dict(nx=5, ny=384, nz=1)),                  # Py = 768
+    ("y2048", dict(nx=4, ny=1024, nz=1)),                # Py = 2048
+    ("yz1024_2layers", dict(nx=8, ny=512, nz=2)),        # direct z, Py = 1024
+]
+
+
+@pytest.mark.parametrize("name,geo", STRIPS, ids=[s[0] for s in STRIPS])
+def test_strip_lengths_fp64(orc, mmc, name, geo):
+    """FP64 field, energy and three-sweep chain against the oracle."""
+    kw = film(**geo)
+    o = orc.Oracle(**kw)
+    o.build_tensor()
+    c = mmc.Context(mmc.make_params(**kw), 0)
+    c.set_tensor(o.tensor())
+    m = random_m((kw["nx"], kw["ny"], kw["nz"]), 70)
+    o.set_m(m)
+    c.set_m(m)
+    _check_state(o, c, 1e-12, 1e-12, 1e-12)
+    _fp64_chain(c, (1400.0, 500.0, 120.0), [o])
+    assert rel_inf(c.m(), o.m()) <= 1e-10
+
+
+@pytest.mark.parametrize("name,geo", STRIPS, ids=[s[0] for s in STRIPS])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_strip_schedules_agree(mmc, name, geo, precision):
+    """Every admissible schedule (fused with one row pair or full tiles,
+    row sweeps, separate launches; eager and graph-replayed) gives the
+    bit-identical chain at every transform length."""
+    kw = film(**geo)
+    ref = None
+    c = mmc.Context(mmc.make_params(**kw), precision)
+    c.build_tensor()
+    names = [ln.rsplit(None, 2)[0] for ln in c.autotune(2).splitlines()
+             if ln.endswith("us/sweep")]
+    K = c.tensor()
+    for nm in names:
+        for graph in (0, 1):
+            b = mmc.Context(mmc.make_params(**kw), precision)
+            b.set_tensor(K)
+            b.init_random_m()
+            b.set_temperature(650.0)
+            b.set_schedule(nm)
+            if graph:
+                b.set_option("graph", 1)
+                b.set_option("graph_sweeps", 3)
+            b.sweeps(4, 0, 7)
+            got = (b.m(), b.energy()["total"])
+            b.close()
+            if ref is None:
+                ref = got
+            assert np.array_equal(got[0], ref[0]), f"{nm} graph={graph}"
+            assert got[1] == ref[1]

@@ -22,6 +22,8 @@ CASES = [
     ("box3d_pbcz", dict(nx=10, ny=8, nz=3, pbc_z=2)),
     ("ragged_3d", dict(nx=7, ny=9, nz=5, pbc_y=1)),
     ("two_layer_3d_zconv", dict(nx=32, ny=32, nz=2)),
+    ("deep_z_3d", dict(nx=16, ny=16, nz=16)),                # Pz = 32, runtime plan
+    ("deep_z_pbcz_3d", dict(nx=12, ny=10, nz=16, pbc_z=1)),  # Pz = 16
 ]
 
 
@@ -187,8 +189,10 @@ def _run(mmc, kw, precision, n, graph=False, tyx=None, ncol=None, chunk=None):
 
 @pytest.mark.parametrize("geo", [dict(nx=32, ny=24, nz=1, pbc_x=2),
                                  dict(nx=16, ny=16, nz=4),
-                                 dict(nx=12, ny=10, nz=5, pbc_y=1)],
-                         ids=["film", "3d_direct_z", "3d_lds_z"])
+                                 dict(nx=12, ny=10, nz=5, pbc_y=1),
+                                 dict(nx=16, ny=16, nz=16),
+                                 dict(nx=12, ny=10, nz=16, pbc_z=1)],
+                         ids=["film", "3d_direct_z", "3d_lds_z", "3d_pz32", "3d_pz16"])
 def test_graph_replay_matches_eager(mmc, geo):
     """hipGraph replays (4 sweeps per graph, replayed with eager work and host
     reads in between) give bit-identical states to eager launches after
@@ -238,9 +242,11 @@ def test_tiled_metropolis_matches_per_colour(mmc, geo, precision):
                                  dict(nx=64, ny=16, nz=4, pbc_z=1),
                                  dict(nx=256, ny=50, nz=1, pbc_x=3),
                                  dict(nx=128, ny=37, nz=2),
-                                 dict(nx=128, ny=10, nz=2, pbc_x=1, pbc_y=1)],
+                                 dict(nx=128, ny=10, nz=2, pbc_x=1, pbc_y=1),
+                                 dict(nx=64, ny=16, nz=16),
+                                 dict(nx=128, ny=10, nz=16, pbc_z=1)],
                          ids=["open", "pbc", "odd_open", "3d_open", "3d_pbcz", "wide_pbc",
-                              "zfold_open", "zfold_pbc"])
+                              "zfold_open", "zfold_pbc", "deep_z_pz32", "deep_z_pz16"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_fused_sweeps_match_separate(mmc, geo, precision):
     """Inverse x pass fused with colour 0 and colour 1 fused with the next

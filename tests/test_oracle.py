@@ -468,3 +468,66 @@ def test_exchange_pair_langevin(orc):
             mm = o.m().reshape(3, 2)
             acc.append((mm[:, 0] * mm[:, 1]).sum())
     assert abs(np.mean(acc) - _langevin(x)) < 0.03
+
+
+# --------------------------------------------------------------------------
+# <m> of the energy routine and the seeded start state
+# --------------------------------------------------------------------------
+def test_mean_magnetisation(orc):
+    """<m> returned by the energy routine: exactly the direction of a
+    uniform state, zero for a helix over whole periods, and numpy's mean of
+    a random state (a slip such as dividing by the padded cell count, or
+    averaging one component twice, fails one of these)."""
+    kw = film(nx=24, ny=16, nz=2, pbc_x=1, pbc_y=1)
+    o = orc.Oracle(**kw)
+    o.build_tensor()
+    u = np.array([0.36, -0.48, 0.8])            # unit length exactly
+    m = np.empty((3, 2, 16, 24))
+    for k in range(3):
+        m[k] = u[k]
+    o.set_m(m)
+    assert np.array_equal(o.energy()["m_avg"], u)
+    x = np.arange(24)
+    q = 2 * np.pi * 3 / 24                      # three whole periods along x
+    m = np.zeros((3, 2, 16, 24))
+    m[0] = np.cos(q * x)[None, None, :]
+    m[2] = np.sin(q * x)[None, None, :]
+    o.set_m(m)
+    assert np.abs(o.energy()["m_avg"]).max() <= 1e-15
+    m = random_m((24, 16, 2), 31)
+    o.set_m(m)
+    ref = m.reshape(3, -1).mean(axis=1)
+    assert np.abs(o.energy()["m_avg"] - ref).max() <= 1e-15
+
+
+def test_random_start_is_uniform_on_the_sphere(orc):
+    """The seeded start state over 2^20 cells: unit vectors whose moments
+    match the uniform distribution on the sphere within 5 standard errors
+    (<m> = 0, <m_z^2> = 1/3, <m_x m_y> = 0) and whose z and azimuth
+    histograms pass a chi-square test; the state is a function of the seed.
+    z = u or phi = pi*u instead of z = 1 - 2u, phi = 2 pi u fail the
+    moments."""
+    from scipy import stats
+    kw = dict(nx=1024, ny=1024, nz=1, pbc_x=-1, pbc_y=-1, cal_demag=0)
+    o = orc.Oracle(**film(**kw))
+    o.init_random_m(4242)
+    m = o.m().reshape(3, -1)
+    N = m.shape[1]
+    assert N == 1 << 20
+    assert np.abs(np.linalg.norm(m, axis=0) - 1.0).max() <= 1e-15
+    se_m = math.sqrt(1.0 / 3.0 / N)
+    assert np.abs(m.mean(axis=1)).max() <= 5 * se_m
+    mz2 = (m[2] ** 2).mean()
+    assert abs(mz2 - 1.0 / 3.0) <= 5 * math.sqrt((1.0 / 5.0 - 1.0 / 9.0) / N)
+    assert abs((m[0] * m[1]).mean()) <= 5 * math.sqrt(1.0 / 15.0 / N)
+    bins = 32
+    hz, _ = np.histogram(m[2], bins=bins, range=(-1.0, 1.0))
+    phi = np.mod(np.arctan2(m[1], m[0]), 2 * np.pi)
+    hp, _ = np.histogram(phi, bins=bins, range=(0.0, 2 * np.pi))
+    for h in (hz, hp):
+        assert stats.chisquare(h).pvalue > 1e-6
+    o2 = orc.Oracle(**film(**kw))
+    o2.init_random_m(4242)
+    assert np.array_equal(o2.m(), o.m())
+    o2.init_random_m(4243)
+    assert not np.array_equal(o2.m(), o.m())
