@@ -1,16 +1,21 @@
 """Benchmark: Metropolis spin-update throughput of the annealing hot path.
 
-Workload (one step = one sweep = one Metropolis attempt per cell, preceded by
-the demag convolution of the pre-sweep state): the 256x256x1 film,
-dx=dy=1 nm, dz=0.8 nm, Ms=1.6e6 A/m, Aex=16 pJ/m, Dind=6 mJ/m^2,
-Ku1=2 MJ/m^3 along z, Bz=0.3 T, periodic in x and y with 20 images each,
-seeded uniform-random start, in FP32-bulk mode (FP32 FFT/stencil/dE, FP64
-tensor and reductions).  The same workload in plain FP64 and the
-1024x1024x2 film (HBM-resident working set) are reported beside it.
+Headline workload (one step = one sweep = one Metropolis attempt per cell of
+every instance, preceded by the demag convolution of the pre-sweep state):
+64 independent anneals batched in one context, nz=1 films n x n with n drawn
+from {64, 96, 128, 192, 256} by a seeded generator, Bz ~ U(0, 0.6) T,
+Dind ~ U(3, 6) mJ/m^2, Ms/Aex/Ku1 of the 64x64 film, open boundaries,
+distinct Philox seeds, FP32-bulk (FP32 FFT/stencil/dE, FP64 tensor and
+reductions), the sweep schedule chosen by the library's autotuner, sweeps
+replayed from hipGraphs.  Beside it: the 64x64 film in plain FP64 (the
+single-instance denominator of the batched/single ratio), the accuracy of
+FP32 bulk over the 64x64 film's full anneal (16 seeds), the 256x256 film
+with 20 periodic images (FP32 and FP64), the 128x128x16 film and the
+1024x1024x2 film.
 
-The timed steps replay hipGraph-captured sweeps; the per-kernel durations
-behind `roofline` come from the sweeps right after them, launched eagerly with
-HIP events recorded by each kernel's dispatch packet on the library's stream.
+Per-kernel durations behind `roofline` come from eager sweeps right after
+the timed ones, each launch bracketed by HIP events that its dispatch packet
+records on the library's stream.
 
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -189,6 +194,28 @@ def batch_params(count=64, master_seed=2024):
     return out
 
 
+def _batch64():
+    kws = batch_params()
+    sizes = sorted({kw["nx"] for kw in kws})
+    counts = {n: sum(kw["nx"] == n for kw in kws) for n in sizes}
+    return kws, ("anneal sweep, 64 independent films batched in one context, n x n x 1 with "
+                 "n drawn from {64,96,128,192,256} (master seed 2024: "
+                 + ", ".join(f"{counts[n]}x{n}^2" for n in sizes)
+                 + "), open boundaries, Bz ~ U(0,0.6) T, Dind ~ U(3,6) mJ/m^2, FP32-bulk, "
+                   "T=1400 K")
+
+
+WORKLOADS = {
+    "batch64": _batch64,
+    "film256": lambda: ([film256()], "anneal sweep, 256x256x1 film, PBC 20x20 images, "
+                                     "FP32-bulk, T=1400 K"),
+    "film128x16": lambda: ([film128x16()], "anneal sweep, 128x128x16 open film, FP32-bulk, "
+                                           "T=1400 K"),
+    "film1024x2": lambda: ([film1024x2()], "anneal sweep, 1024x1024x2 film, PBC 10x10 images, "
+                                           "FP32-bulk, T=1400 K"),
+}
+
+
 def copy_calibration(nbytes, times):
     """Device-to-device copies of a known size through the HIP runtime the
     library links (hipMemcpy runs the runtime's copy kernel): the reference
@@ -235,24 +262,83 @@ def roofline(kern, workload=None):
                      bytes_per_launch=k["bytes_per_launch"])
 
 
-def cpu_baseline(mod, ctx, kw, sweeps):
-    """FP64 oracle sweeps of the same film (tensor taken from the GPU build)."""
+def cpu_baseline(mod, ctx, kws, sweeps):
+    """FP64 oracle sweeps of the same instances (each instance's tensor taken
+    from the GPU build; one instance at a time), on all host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    o = oracle.Oracle(**kw)
-    o.set_tensor(ctx.tensor())
-    o.init_random_m(kw["seed"])
-    o.sweep(1400.0, 0, 0)       # builds the tensor spectrum outside the timed part
-    t0 = time.perf_counter()
-    for s in range(sweeps):
-        o.sweep(1400.0, 0, 1 + s)
-    dt = time.perf_counter() - t0
-    N = kw["nx"] * kw["ny"] * kw["nz"]
+    attempts, dt = 0, 0.0
+    for i, kw in enumerate(kws):
+        o = oracle.Oracle(**kw)
+        o.set_tensor(ctx.tensor(i))
+        o.init_random_m(kw["seed"])
+        o.sweep(1400.0, 0, 0)       # builds the tensor spectrum outside the timed part
+        t0 = time.perf_counter()
+        for s in range(sweeps):
+            o.sweep(1400.0, 0, 1 + s)
+        dt += time.perf_counter() - t0
+        attempts += kw["nx"] * kw["ny"] * kw["nz"] * sweeps
+        del o
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=N * sweeps / dt, unit="spin-update attempts/s", cores=cores,
-                kind="port",
-                sample=f"{sweeps} FP64 sweeps of the {kw['nx']}x{kw['ny']}x{kw['nz']} "
-                       f"PBC film on the C oracle (OpenMP), GPU-built tensor")
+    what = (f"{len(kws)} instances" if len(kws) > 1
+            else f"the {kws[0]['nx']}x{kws[0]['ny']}x{kws[0]['nz']} film")
+    return dict(value=attempts / dt, unit="spin-update attempts/s", cores=cores, kind="port",
+                sample=f"{sweeps} FP64 sweeps of each of {what} ({attempts} attempts, "
+                       f"{dt:.1f} s) on the C oracle (OpenMP), GPU-built tensors")
+
+
+def anneal_stages(T0=1400.0, T1=40.0, f=0.9):
+    out, T = [], T0
+    while T > T1:
+        out.append(T)
+        T *= f
+    return out
+
+
+def equal_accuracy(mod, seeds=16, sweeps_per_stage=1000):
+    """The 64x64 film annealed on its full schedule (1400 K x0.9 to 40 K, 34
+    stages) for `seeds` seeds, batched, in FP64 and in FP32 bulk: seed means
+    and standard errors of the final energy density and <m_z>, and the FP32
+    path's final-energy error (its energy of its final states against the FP64
+    path's energy of the same states; the FP64 path equals the FP64 oracle
+    within 1e-12 in the parity tests)."""
+    import numpy as np
+    kws = [film256(nx=64, ny=64, pbc_x=0, pbc_y=0, seed=s) for s in range(1, seeds + 1)]
+    V = 64 * 64 * 1e-9 * 1e-9 * 0.8e-9
+    out, finals = {}, {}
+    t0 = time.perf_counter()
+    for prec, name in ((mod.FP64, "fp64"), (mod.FP32, "fp32")):
+        b = mod.Context([mod.make_params(**kw) for kw in kws], prec)
+        b.build_tensor()
+        b.init_random_m()
+        b.set_option("graph", 1)
+        b.set_option("graph_sweeps", 20)
+        for st, T in enumerate(anneal_stages()):
+            b.set_temperature(T)
+            b.sweeps(st, 0, sweeps_per_stage)
+        e = np.array([b.energy(i)["total"] / V for i in range(seeds)])
+        mz = np.array([b.energy(i)["m_avg"][2] for i in range(seeds)])
+        finals[name] = (e, [b.m(i) for i in range(seeds)])
+        out[name] = dict(energy_density_mean=float(e.mean()),
+                         energy_density_se=float(e.std(ddof=1) / np.sqrt(seeds)),
+                         mz_mean=float(mz.mean()), mz_se=float(mz.std(ddof=1) / np.sqrt(seeds)))
+        b.close()
+    c64 = mod.Context([mod.make_params(**kw) for kw in kws], mod.FP64)
+    c64.build_tensor()
+    worst = 0.0
+    for i in range(seeds):
+        c64.set_m(finals["fp32"][1][i], i)
+    for i in range(seeds):
+        e64 = c64.energy(i)["total"] / V
+        worst = max(worst, abs(finals["fp32"][0][i] - e64) / abs(e64))
+    c64.close()
+    d = out["fp64"]["energy_density_mean"] - out["fp32"]["energy_density_mean"]
+    se = (out["fp64"]["energy_density_se"] ** 2 + out["fp32"]["energy_density_se"] ** 2) ** 0.5
+    out.update(workload=f"64x64 film, {seeds} seeds, 34 stages x {sweeps_per_stage} sweeps",
+               fp32_final_energy_rel_err=worst,
+               mean_energy_density_diff_in_se=abs(d) / se if se > 0 else 0.0,
+               seconds=time.perf_counter() - t0)
+    return out
 
 
 def main():
@@ -260,11 +346,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-sweeps", type=int, default=20)
+    ap.add_argument("--cpu-sweeps", type=int, default=10,
+                    help="oracle sweeps per instance for the CPU baseline (0: skip it)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the FP64 and 1024x1024x2 side measurements")
-    ap.add_argument("--workload", choices=("film256", "film1024x2"), default="film256",
-                    help="film measured as `value` (film1024x2 only for profiling runs)")
+                    help="skip the side measurements (other films, accuracy anneal)")
+    ap.add_argument("--workload", choices=tuple(WORKLOADS), default="batch64",
+                    help="workload measured as `value` (others only for profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the timed sweeps eagerly instead of replaying hipGraphs")
     ap.add_argument("--default-stream", action="store_true",
@@ -274,7 +361,7 @@ def main():
                     help="sweeps per candidate schedule for the library's autotuner "
                          "(0: keep the default schedule)")
     ap.add_argument("--schedule", default=None,
-                    help="apply this named sweep schedule to the measured film instead of "
+                    help="apply this named sweep schedule to the measured workload instead of "
                          "autotuning (profiling runs)")
     ap.add_argument("--ksteps", type=int, default=None,
                     help="event-timed eager sweeps after the timed region (default "
@@ -292,52 +379,51 @@ def main():
         copy_calibration(268435456, 3)
         log("[bench] calibration: 3 device copies of 268435456 bytes")
 
-    kw = film256() if a.workload == "film256" else film1024x2()
+    kws, desc = WORKLOADS[a.workload]()
     log(f"[bench] {a.workload} FP32-bulk: {a.warmup} warmup + {a.steps} timed sweeps")
-    r32 = run_gpu(mod, kw, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps,
+    r32 = run_gpu(mod, kws, mod.FP32, a.steps, a.warmup, graph=not a.no_graph, ksteps=a.ksteps,
                   default_stream=a.default_stream,
                   autotune=a.schedule if a.schedule else a.autotune)
     dom, roof = roofline(r32["kernels"], a.workload)
     extra = {}
     if not a.no_extra:
-        log("[bench] same film, plain FP64")
-        r64 = run_gpu(mod, kw, mod.FP64, max(200, a.steps // 2), max(20, a.warmup // 4),
-                      autotune=a.autotune)
-        extra["fp64_attempts_per_s"] = r64["attempts_per_s"]
-        extra["fp64_ms_per_sweep"] = r64["ms_per_sweep"]
-        extra["fp64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r64["kernels"].items()}
-        extra["fp32_over_fp64"] = r32["attempts_per_s"] / r64["attempts_per_s"]
-        r64["ctx"].close()
         log("[bench] 64x64x1 open film, plain FP64 single instance")
         kw0 = film256(nx=64, ny=64, pbc_x=0, pbc_y=0)
         r0 = run_gpu(mod, kw0, mod.FP64, max(400, a.steps), a.warmup, autotune=a.autotune)
         extra["film64_fp64_attempts_per_s"] = r0["attempts_per_s"]
         extra["film64_fp64_ms_per_sweep"] = r0["ms_per_sweep"]
+        extra["film64_fp64_schedule"] = r0["schedule"]
+        if a.workload == "batch64":
+            extra["batched_fp32_over_single_fp64"] = r32["attempts_per_s"] / r0["attempts_per_s"]
         r0["ctx"].close()
-        log("[bench] 64 uneven films batched, FP32-bulk")
-        bp = batch_params()
-        rb = run_gpu(mod, bp, mod.FP32, 200, 20, autotune=a.autotune)
-        db, roofb = roofline(rb["kernels"])
-        extra["batch64_fp32_attempts_per_s"] = rb["attempts_per_s"]
-        extra["batch64_ms_per_sweep"] = rb["ms_per_sweep"]
-        extra["batch64_cells"] = rb["cells"]
-        extra["batch64_roofline"] = roofb
-        extra["batch64_schedule"] = rb["schedule"]
-        extra["batch64_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in rb["kernels"].items()}
-        extra["batched_fp32_over_single_fp64"] = rb["attempts_per_s"] / r0["attempts_per_s"]
-        rb["ctx"].close()
+        log("[bench] equal accuracy: 16 seeds of the 64x64 film's anneal, FP64 and FP32 bulk")
+        extra["equal_accuracy"] = equal_accuracy(mod)
+        kw1 = film256()
+        log("[bench] 256x256 PBC20 film, FP32 bulk and plain FP64")
+        r1 = run_gpu(mod, kw1, mod.FP32, max(200, a.steps // 4), max(20, a.warmup // 4),
+                     autotune=a.autotune)
+        extra["film256_fp32_attempts_per_s"] = r1["attempts_per_s"]
+        extra["film256_fp32_ms_per_sweep"] = r1["ms_per_sweep"]
+        extra["film256_fp32_schedule"] = r1["schedule"]
+        extra["film256_fp32_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r1["kernels"].items()}
+        extra["film256_fp32_error_vs_fp64"] = fp32_errors(mod, kw1, r1["ctx"])
+        r1["ctx"].close()
+        r64 = run_gpu(mod, kw1, mod.FP64, max(200, a.steps // 4), max(20, a.warmup // 4),
+                      autotune=a.autotune)
+        extra["film256_fp64_attempts_per_s"] = r64["attempts_per_s"]
+        extra["film256_fp64_ms_per_sweep"] = r64["ms_per_sweep"]
+        r64["ctx"].close()
         log("[bench] 128x128x16 open 3-D film FP32-bulk")
         r2 = run_gpu(mod, film128x16(), mod.FP32, 200, 20, autotune=a.autotune)
+        d2, roof2 = roofline(r2["kernels"], "film128x16")
         extra["film128x16_attempts_per_s"] = r2["attempts_per_s"]
         extra["film128x16_ms_per_sweep"] = r2["ms_per_sweep"]
         extra["film128x16_schedule"] = r2["schedule"]
+        extra["film128x16_roofline"] = roof2
         extra["film128x16_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r2["kernels"].items()}
         r2["ctx"].close()
-        log("[bench] FP32-bulk error against FP64 on the 256x256 film")
-        extra["fp32_error_vs_fp64"] = fp32_errors(mod, kw, r32["ctx"])
-        kw4 = film1024x2()
         log("[bench] 1024x1024x2 PBC10 FP32-bulk")
-        r4 = run_gpu(mod, kw4, mod.FP32, 200, 20, autotune=a.autotune)
+        r4 = run_gpu(mod, film1024x2(), mod.FP32, 200, 20, autotune=a.autotune)
         d4, roof4 = roofline(r4["kernels"], "film1024x2")
         extra["film1024x2_attempts_per_s"] = r4["attempts_per_s"]
         extra["film1024x2_ms_per_sweep"] = r4["ms_per_sweep"]
@@ -346,8 +432,10 @@ def main():
         extra["film1024x2_schedule"] = r4["schedule"]
         extra["film1024x2_kernels_us"] = {k: round(v["avg_us"], 3) for k, v in r4["kernels"].items()}
         r4["ctx"].close()
-    log("[bench] CPU baseline (oracle)")
-    cpu = cpu_baseline(mod, r32["ctx"], kw, a.cpu_sweeps)
+    cpu = None
+    if a.cpu_sweeps > 0:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(mod, r32["ctx"], kws, a.cpu_sweeps)
     line = {
         "metric": METRIC,
         "value": r32["attempts_per_s"],
@@ -361,10 +449,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": (f"anneal sweep, {kw['nx']}x{kw['ny']}x{kw['nz']} film, "
-                                f"PBC {kw['pbc_x']}x{kw['pbc_y']} images, FP32-bulk, T=1400 K"),
-                   "cells": kw["nx"] * kw["ny"] * kw["nz"], "step": "one sweep",
-                   "precision": "fp32-bulk"},
+        "config": {"workload": desc, "cells": r32["cells"], "instances": len(kws),
+                   "step": "one sweep of every instance", "precision": "fp32-bulk"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in r32["kernels"].items()},

@@ -12,8 +12,8 @@ namespace mmc {
 // does (zf = 0), or transformed along z in registers and stored to the
 // xz-spectrum B as k_zfwd does (zf = 1).
 template <typename T, int N, int RZ>
-__global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict__ I,
-                                                         const Work *__restrict__ W,
+__device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
+                                                         const Work w,
                                                          T *__restrict__ mb,
                                                          const T *__restrict__ hb,
                                                          cpx<T> *__restrict__ specb,
@@ -24,7 +24,6 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
 {
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     cpx<T> *tw = s + 3 * RZ * N;
-    const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
     const bool p16 = pair16<T>(in);
     const int z0 = w.b, y0 = w.c;
@@ -190,6 +189,44 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
     }
 }
 
+template <typename T, int N, int RZ>
+__global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict__ I,
+                                                         const Work *__restrict__ W,
+                                                         T *__restrict__ mb,
+                                                         const T *__restrict__ hb,
+                                                         cpx<T> *__restrict__ specb,
+                                                         const double *__restrict__ beta,
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint32_t sweep_off,
+                                                         const cpx<T> *__restrict__ twb)
+{
+    metro_xfwd_tile<T, N, RZ>(I, W[blockIdx.x], mb, hb, specb, beta, ctl, sweep_off, twb);
+}
+
+// One launch over items of every x length of a batch (any_x_len), tile
+// policy POL (any_tile).
+template <typename T, int POL>
+__global__ __launch_bounds__(kThreads) void k_metro_xfwd_any(const Inst *__restrict__ I,
+                                                             const Work *__restrict__ W,
+                                                             T *__restrict__ mb,
+                                                             const T *__restrict__ hb,
+                                                             cpx<T> *__restrict__ specb,
+                                                             const double *__restrict__ beta,
+                                                             const uint32_t *__restrict__ ctl,
+                                                             uint32_t sweep_off,
+                                                             const cpx<T> *__restrict__ twb)
+{
+    const Work w = W[blockIdx.x];
+    switch (I[w.inst].Px) {
+#define MMC_ANY(NN)                                                                              \
+    case NN:                                                                                     \
+        metro_xfwd_tile<T, NN, any_tile(POL, NN)>(I, w, mb, hb, specb, beta, ctl, sweep_off, twb); \
+        break;
+    MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
+#undef MMC_ANY
+    }
+}
+
 template <typename T>
 hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
                              int rows, T *m, const T *h, cpx<T> *spec, const double *beta,
@@ -198,6 +235,15 @@ hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n
     if (n <= 0) return hipSuccess;
     const dim3 g(n), b(kThreads);
     constexpr int F = fused_rows_base<T>();
+    if (fixed_n == kAnyLen) {
+        switch (rows) {
+        case 0: MMC_LAUNCH((k_metro_xfwd_any<T, 0>), g, b, lds, st, I, W, m, h, spec, beta, ctl, sweep_off, tw); break;
+        case 1: MMC_LAUNCH((k_metro_xfwd_any<T, 1>), g, b, lds, st, I, W, m, h, spec, beta, ctl, sweep_off, tw); break;
+        case 2: MMC_LAUNCH((k_metro_xfwd_any<T, 2>), g, b, lds, st, I, W, m, h, spec, beta, ctl, sweep_off, tw); break;
+        default: MMC_LAUNCH((k_metro_xfwd_any<T, 3>), g, b, lds, st, I, W, m, h, spec, beta, ctl, sweep_off, tw); break;
+        }
+        return post();
+    }
 #define MMC_MX_CASE(TT, NN, RR)                                                                  \
     if (fixed_n == NN && rows == RR) {                                                           \
         MMC_LAUNCH((k_metro_xfwd<TT, NN, RR>), g, b, lds, st, I, W, m, h, spec, beta, ctl,       \
@@ -217,6 +263,10 @@ hipError_t set_lds_fused_fwd(int bytes)
 #define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_metro_xfwd<TT, NN, RR>, bytes);
     MMC_FUSED(MMC_FL, T, F)
 #undef MMC_FL
+    set_lds_one(e, (const void *)k_metro_xfwd_any<T, 0>, bytes);
+    set_lds_one(e, (const void *)k_metro_xfwd_any<T, 1>, bytes);
+    set_lds_one(e, (const void *)k_metro_xfwd_any<T, 2>, bytes);
+    set_lds_one(e, (const void *)k_metro_xfwd_any<T, 3>, bytes);
     return e;
 }
 

@@ -17,8 +17,8 @@ namespace mmc {
 // hw = ceil(nx/2), in the instance's h region.  Same arithmetic as
 // k_zinv + k_xinv_n + k_metro.
 template <typename T, int N, int RZ>
-__global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict__ I,
-                                                         const Work *__restrict__ W,
+__device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
+                                                         const Work w,
                                                          const cpx<T> *__restrict__ specb,
                                                          T *__restrict__ mb, T *__restrict__ hb,
                                                          const double *__restrict__ beta,
@@ -28,7 +28,6 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
 {
     cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
     cpx<T> *tw = s + 3 * RZ * N;
-    const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
     const bool p16 = pair16<T>(in);
     const int z0 = w.b, y0 = w.c;
@@ -183,6 +182,42 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
     }
 }
 
+template <typename T, int N, int RZ>
+__global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict__ I,
+                                                         const Work *__restrict__ W,
+                                                         const cpx<T> *__restrict__ specb,
+                                                         T *__restrict__ mb, T *__restrict__ hb,
+                                                         const double *__restrict__ beta,
+                                                         const uint32_t *__restrict__ ctl,
+                                                         uint32_t sweep_off,
+                                                         const cpx<T> *__restrict__ twb)
+{
+    xinv_metro_tile<T, N, RZ>(I, W[blockIdx.x], specb, mb, hb, beta, ctl, sweep_off, twb);
+}
+
+// One launch over items of every x length of a batch (any_x_len), tile
+// policy POL (any_tile).
+template <typename T, int POL>
+__global__ __launch_bounds__(kThreads) void k_xinv_metro_any(const Inst *__restrict__ I,
+                                                             const Work *__restrict__ W,
+                                                             const cpx<T> *__restrict__ specb,
+                                                             T *__restrict__ mb, T *__restrict__ hb,
+                                                             const double *__restrict__ beta,
+                                                             const uint32_t *__restrict__ ctl,
+                                                             uint32_t sweep_off,
+                                                             const cpx<T> *__restrict__ twb)
+{
+    const Work w = W[blockIdx.x];
+    switch (I[w.inst].Px) {
+#define MMC_ANY(NN)                                                                              \
+    case NN:                                                                                     \
+        xinv_metro_tile<T, NN, any_tile(POL, NN)>(I, w, specb, mb, hb, beta, ctl, sweep_off, twb); \
+        break;
+    MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
+#undef MMC_ANY
+    }
+}
+
 template <typename T>
 hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
                              int rows, const cpx<T> *spec, T *m, T *h, const double *beta,
@@ -191,6 +226,15 @@ hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n
     if (n <= 0) return hipSuccess;
     const dim3 g(n), b(kThreads);
     constexpr int F = fused_rows_base<T>();
+    if (fixed_n == kAnyLen) {
+        switch (rows) {
+        case 0: MMC_LAUNCH((k_xinv_metro_any<T, 0>), g, b, lds, st, I, W, spec, m, h, beta, ctl, sweep_off, tw); break;
+        case 1: MMC_LAUNCH((k_xinv_metro_any<T, 1>), g, b, lds, st, I, W, spec, m, h, beta, ctl, sweep_off, tw); break;
+        case 2: MMC_LAUNCH((k_xinv_metro_any<T, 2>), g, b, lds, st, I, W, spec, m, h, beta, ctl, sweep_off, tw); break;
+        default: MMC_LAUNCH((k_xinv_metro_any<T, 3>), g, b, lds, st, I, W, spec, m, h, beta, ctl, sweep_off, tw); break;
+        }
+        return post();
+    }
 #define MMC_XM_CASE(TT, NN, RR)                                                                  \
     if (fixed_n == NN && rows == RR) {                                                           \
         MMC_LAUNCH((k_xinv_metro<TT, NN, RR>), g, b, lds, st, I, W, spec, m, h, beta, ctl,       \
@@ -210,6 +254,10 @@ hipError_t set_lds_fused_inv(int bytes)
 #define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_xinv_metro<TT, NN, RR>, bytes);
     MMC_FUSED(MMC_FL, T, F)
 #undef MMC_FL
+    set_lds_one(e, (const void *)k_xinv_metro_any<T, 0>, bytes);
+    set_lds_one(e, (const void *)k_xinv_metro_any<T, 1>, bytes);
+    set_lds_one(e, (const void *)k_xinv_metro_any<T, 2>, bytes);
+    set_lds_one(e, (const void *)k_xinv_metro_any<T, 3>, bytes);
     return e;
 }
 

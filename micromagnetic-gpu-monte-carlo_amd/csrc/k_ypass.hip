@@ -234,6 +234,24 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
     ymul_fixed<T, N, C>(I[w.inst], w, bufb, khatb, twb);
 }
 
+// One launch over y-pass items of every y length of a batch (any_y_len),
+// columns per item from tile policy POL (any_tile).
+template <typename T, int POL>
+__global__ __launch_bounds__(kThreads) void k_ymul_any(const Inst *__restrict__ I,
+                                                       const Work *__restrict__ W,
+                                                       cpx<T> *__restrict__ bufb,
+                                                       const cpx<T> *__restrict__ khatb,
+                                                       const cpx<T> *__restrict__ twb)
+{
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    switch (in.Py) {
+#define MMC_ANY(NN) case NN: ymul_fixed<T, NN, any_tile(POL, NN)>(in, w, bufb, khatb, twb); break;
+    MMC_ANY(64) MMC_ANY(96) MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
+#undef MMC_ANY
+    }
+}
+
 // y forward transform, k-space product with a direct z convolution, y
 // inverse of one x frequency of an instance with a few layers (nz = NZ <=
 // kZConvMax, Pz <= kZDirect, Py = N).  The tensor's z slices are transformed
@@ -482,6 +500,15 @@ hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int 
 {
     if (n <= 0) return hipSuccess;
     const dim3 g(n), b(kThreads);
+    if (fixed_n == kAnyLen) {
+        switch (cols) {
+        case 0: MMC_LAUNCH((k_ymul_any<T, 0>), g, b, lds, st, I, W, buf, khat, tw); break;
+        case 1: MMC_LAUNCH((k_ymul_any<T, 1>), g, b, lds, st, I, W, buf, khat, tw); break;
+        case 2: MMC_LAUNCH((k_ymul_any<T, 2>), g, b, lds, st, I, W, buf, khat, tw); break;
+        default: MMC_LAUNCH((k_ymul_any<T, 3>), g, b, lds, st, I, W, buf, khat, tw); break;
+        }
+        return post();
+    }
 #define MMC_YN(NN)                                                                              \
     if (fixed_n == NN) {                                                                        \
         if (cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1>), g, b, lds, st, I, W, buf, khat, tw);   \
@@ -537,6 +564,10 @@ hipError_t set_lds_ypass(int bytes)
     MMC_YL(768) MMC_YL(1024) MMC_YL(2048)
 #undef MMC_YL
     set_lds_one(e, (const void *)k_yfwd_tensor<T>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 0>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 1>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 2>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 3>, bytes);
 #define MMC_YZL(NN) set_lds_one(e, (const void *)k_yzconv_n<T, NN, 2>, bytes); \
     set_lds_one(e, (const void *)k_yzconv_n<T, NN, 3>, bytes);                 \
     set_lds_one(e, (const void *)k_yzconv_n<T, NN, 4>, bytes);

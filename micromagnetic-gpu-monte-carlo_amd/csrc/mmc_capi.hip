@@ -129,6 +129,9 @@ struct mmc_ctx {
     bool fuse = false;                // sweeps run the fused x + Metropolis passes
     bool fuse_ok = false;             // every instance admits them
     bool zfold = false;               // fused tiles of short-z instances fold z in
+    int any_pol = -1;                 // >= 0: the y pass and both fused passes are one
+                                      // launch each over every instance (tile policy)
+    bool any_ok = false;              // every instance admits those launches
     bool concurrent = false;          // per-length launches of a pass on side streams
     std::vector<hipStream_t> side;    // side streams (default priority) and the
     hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
@@ -487,6 +490,30 @@ int line_share(int rows, size_t cs)
     return chunk >= 128 ? 1 : 128 / chunk;
 }
 
+// One segment (one launch of an "any" kernel) holding every segment's
+// items, the longest transforms first so that the short items fill the
+// launch's tail; each segment's XCD grouping survives the concatenation
+// (a shift of the start index shifts every item of a group alike).
+void merge_segments(List &l, int pol)
+{
+    if (l.seg.empty()) return;
+    std::vector<Segment> segs = l.seg;
+    std::stable_sort(segs.begin(), segs.end(),
+                     [](const Segment &a, const Segment &b) { return a.fixed_n > b.fixed_n; });
+    Segment m;
+    m.fixed_n = kAnyLen;
+    m.rows = pol;
+    std::vector<Work> items;
+    for (const Segment &sg : segs) {
+        items.insert(items.end(), l.host.begin() + sg.begin, l.host.begin() + sg.begin + sg.count);
+        m.lds = std::max(m.lds, sg.lds);
+        m.bytes += sg.bytes;
+    }
+    m.count = (int)items.size();
+    l.host.swap(items);
+    l.seg.assign(1, m);
+}
+
 void build_lists(mmc_ctx *c)
 {
     const size_t cs = 2 * c->tsize;   // complex element size
@@ -649,6 +676,16 @@ void build_lists(mmc_ctx *c)
         c->yl.host.insert(c->yl.host.end(), kv.second.begin(), kv.second.end());
         c->yl.seg.push_back(sg);
     }
+    // the "any" kernels take their tile shape from the policy: a tile option
+    // set since (fuse_rows, ncol, zfold) returns the batch to per-length launches
+    for (auto &in : c->inst)
+        if (c->any_pol >= 0 && (in.ncol != any_tile(c->any_pol, in.Py) ||
+                                in.tyf != any_tile(c->any_pol, in.Px) || in.zf))
+            c->any_pol = -1;
+    if (c->any_pol >= 0) {
+        merge_segments(c->yl, c->any_pol);
+        merge_segments(c->fl, c->any_pol);
+    }
 }
 
 template <typename T>
@@ -753,6 +790,9 @@ void adapt_tiles(mmc_ctx *c)
             return true;
         });
     c->fuse = c->fuse_ok;
+    c->any_ok = c->fuse_ok;
+    for (auto &in : c->inst)
+        c->any_ok = c->any_ok && any_x_len(in.Px) && any_y_len(in.Py) && !in.zd;
 }
 
 int alloc_all(mmc_ctx *c)
@@ -1370,6 +1410,8 @@ struct Schedule {
     bool zfold;         // fused tiles of short-z instances fold z in
     bool conc;          // per-length launches of a pass on side streams
     bool rows;          // fused sweeps as the y pass + whole-sweep row-pair tiles
+    int any;            // >= 0: y pass and fused passes one launch each over all
+                        // instances, tile policy `any` (any_tile); -1: per length
 };
 
 // Fused tiles that fold z in where an instance admits them (zfold), else
@@ -1399,7 +1441,12 @@ int apply_schedule(mmc_ctx *c, const Schedule &s)
             in.ncol = 1;
             in.tyf = in.tyf && !in.zf ? 1 : in.tyf;
         }
+        if (s.any >= 0 && c->any_ok) {
+            in.ncol = any_tile(s.any, in.Py);
+            in.tyf = any_tile(s.any, in.Px);
+        }
     }
+    c->any_pol = c->any_ok ? s.any : -1;
     c->fuse = s.fuse;
     c->metro_tiled = s.tiled;
     c->concurrent = s.conc;
@@ -1437,21 +1484,25 @@ int time_sweeps(mmc_ctx *c, int sweeps, double *us)
 }
 
 const Schedule kSchedules[] = {
-        {"fused, launch-sized tiles", true, false, 0, false, false, false},
-        {"fused, minimal tiles", true, false, 1, false, false, false},
-        {"fused, widest tiles", true, false, 2, false, false, false},
-        {"fused, z folded", true, false, 0, true, false, false},
-        {"separate, launch-sized tiles", false, false, 0, false, false, false},
-        {"separate, minimal tiles", false, false, 1, false, false, false},
-        {"separate, widest tiles", false, false, 2, false, false, false},
-        {"tiled two-colour Metropolis", false, true, 0, false, false, false},
-        {"fused, launch-sized, concurrent", true, false, 0, false, true, false},
-        {"fused, minimal, concurrent", true, false, 1, false, true, false},
-        {"separate, launch-sized, concurrent", false, false, 0, false, true, false},
-        {"separate, minimal, concurrent", false, false, 1, false, true, false},
-        {"row sweeps, launch-sized tiles", true, false, 0, false, false, true},
-        {"row sweeps, minimal tiles", true, false, 1, false, false, true},
-        {"row sweeps, minimal, concurrent", true, false, 1, false, true, true},
+        {"fused, launch-sized tiles", true, false, 0, false, false, false, -1},
+        {"fused, minimal tiles", true, false, 1, false, false, false, -1},
+        {"fused, widest tiles", true, false, 2, false, false, false, -1},
+        {"fused, z folded", true, false, 0, true, false, false, -1},
+        {"separate, launch-sized tiles", false, false, 0, false, false, false, -1},
+        {"separate, minimal tiles", false, false, 1, false, false, false, -1},
+        {"separate, widest tiles", false, false, 2, false, false, false, -1},
+        {"tiled two-colour Metropolis", false, true, 0, false, false, false, -1},
+        {"fused, launch-sized, concurrent", true, false, 0, false, true, false, -1},
+        {"fused, minimal, concurrent", true, false, 1, false, true, false, -1},
+        {"separate, launch-sized, concurrent", false, false, 0, false, true, false, -1},
+        {"separate, minimal, concurrent", false, false, 1, false, true, false, -1},
+        {"row sweeps, launch-sized tiles", true, false, 0, false, false, true, -1},
+        {"row sweeps, minimal tiles", true, false, 1, false, false, true, -1},
+        {"row sweeps, minimal, concurrent", true, false, 1, false, true, true, -1},
+        {"one launch per pass, minimal tiles", true, false, 1, false, false, false, 0},
+        {"one launch per pass, paired short rows", true, false, 1, false, false, false, 1},
+        {"one launch per pass, paired rows to 256", true, false, 1, false, false, false, 2},
+        {"one launch per pass, four short rows", true, false, 1, false, false, false, 3},
 };
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
@@ -1480,7 +1531,7 @@ bool schedule_useful(mmc_ctx *c, const Schedule &s)
         c->zfold = z;
     }
     return !((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.zfold && !any_zf) ||
-             (s.conc && !multi) || (s.rows && !c->rows_ok));
+             (s.conc && !multi) || (s.rows && !c->rows_ok) || (s.any >= 0 && (!c->any_ok || !multi)));
 }
 
 template <typename T>
@@ -1862,7 +1913,7 @@ int mmc_set_schedule(mmc_ctx *c, const char *name)
     for (int k = 0; k < kNumSchedules; ++k)
         if (!strcmp(name, kSchedules[k].name)) {
             const Schedule &s = kSchedules[k];
-            if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok))
+            if ((s.fuse && !c->fuse_ok) || (s.tiled && !c->metro_tiled_ok) || (s.any >= 0 && !c->any_ok))
                 return fail(c, MMC_EUNSUP, "schedule '%s' does not apply to these instances", name);
             HIPCHK(c, hipStreamSynchronize(c->stream));
             return apply_schedule(c, s);

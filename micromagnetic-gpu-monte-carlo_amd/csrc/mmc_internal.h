@@ -153,6 +153,26 @@ hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n
                              cpx<T> *spec, const double *beta,
                              const uint32_t *ctl, uint32_t sweep_off,
                              const cpx<T> *tw, size_t lds);
+// One launch per pass over a batch of uneven instances: the "any" kernels
+// read each work item's transform length and run that length's
+// compile-time plan (launch_ymul / launch_xinv_metro / launch_metro_xfwd
+// with fixed_n = kAnyLen; `rows` / `cols` is then the tile policy).
+// Tile policy 0: one row pair (x) or one column (y) per item; 1: two for
+// lengths below 256, whose single rows leave most of a workgroup idle; 2:
+// two up to 256; 3: four below 256, two at 256.
+constexpr int kAnyLen = -1;
+__host__ __device__ constexpr bool any_x_len(int n)
+{
+    return n == 128 || n == 192 || n == 256 || n == 384 || n == 512;
+}
+__host__ __device__ constexpr bool any_y_len(int n) { return n == 64 || n == 96 || any_x_len(n); }
+__host__ __device__ constexpr int any_tile(int pol, int n)
+{
+    return pol == 1 ? (n < 256 ? 2 : 1)
+         : pol == 2 ? (n <= 256 ? 2 : 1)
+         : pol == 3 ? (n < 256 ? 4 : (n == 256 ? 2 : 1)) : 1;
+}
+constexpr int kAnyPolicies = 4;
 // Threads of a k_row_sweep workgroup: a tile is alone on its CU (small films
 // have fewer tiles than CUs; large ones fill its LDS), so its Metropolis
 // cells and butterflies are spread over sixteen waves to keep their

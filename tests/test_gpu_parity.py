@@ -636,3 +636,41 @@ def test_grid_index_limits(mmc):
     for geo in (dict(nx=1 << 24, ny=1, nz=1), dict(nx=1, ny=1 << 12, nz=1 << 12)):
         with pytest.raises(mmc.MMCError, match="2\\^24"):
             mmc.Context(mmc.make_params(**film(**geo)), 1)
+
+
+@pytest.mark.parametrize("graph", [0, 1], ids=["eager", "graph"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_one_launch_per_pass_matches_per_length(mmc, precision, graph):
+    """A batch of x lengths 128..512 and y lengths 64..512 (one periodic,
+    one 3-D instance with z passes) swept with the y pass and both fused
+    passes as one launch each over all instances (each item running its own
+    length's transform), with one or two rows per short item, gives the
+    bit-identical chains, fields and energies of per-length launches."""
+    geos = [film(nx=64, ny=64, seed=1), film(nx=96, ny=48, seed=2),
+            film(nx=128, ny=32, pbc_x=1, seed=3), film(nx=192, ny=32, seed=4),
+            film(nx=256, ny=48, seed=5), film(nx=64, ny=32, nz=5, seed=6),
+            film(nx=256, ny=256, seed=7)]
+    out = []
+    for sched in ("fused, minimal tiles", "one launch per pass, minimal tiles",
+                  "one launch per pass, paired short rows"):
+        b = mmc.Context([mmc.make_params(**kw) for kw in geos], precision)
+        b.build_tensor()
+        b.init_random_m()
+        b.set_temperature(600.0)
+        b.set_schedule(sched)
+        if graph:
+            b.set_option("graph", 1)
+            b.set_option("graph_sweeps", 3)
+        b.set_option("timing", 1)
+        b.sweeps(4, 0, 7)
+        n_y, _, _ = b.kernel_stats("ymul", reset=True)
+        n_x, _, _ = b.kernel_stats("xinv_metro", reset=True)
+        if sched.startswith("one launch") and not graph:
+            assert (n_y, n_x) == (7, 7)
+        out.append(([b.m(i) for i in range(len(geos))], b.field(31, 5),
+                    [b.energy(i)["total"] for i in range(len(geos))]))
+    for ms, h, e in out[1:]:
+        for i, (a, c) in enumerate(zip(ms, out[0][0])):
+            assert np.array_equal(a, c), f"instance {i}"
+        assert np.array_equal(h, out[0][1])
+        assert e == out[0][2]
