@@ -141,6 +141,22 @@ int mmc_field(mmc_ctx *ctx, int inst, int terms, double *hx, double *hy,
 int mmc_energy(mmc_ctx *ctx, int inst, double *E_total, double e_terms[5],
                double m_avg[3]);
 
+/* mmc_energy without device memory of its own: the same energy terms and
+ * mean magnetisation, bit for bit (same kernels, same summation order),
+ * with the reduction's scratch (per-block partial sums and per-instance
+ * sums) in a fixed 2 MiB device array of the library instead of memory the
+ * context allocates; it allocates nothing and does not read or fill the
+ * energy cache.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid allocation).  Limits: at most 1024 instances and at
+ * most 15872 reduction blocks of 256 cells (about 4.06 M cells over all
+ * instances); MMC_EINVAL beyond.  Every call on one GPU uses that GPU's
+ * array, so calls that overlap in time on one GPU (contexts driven from
+ * several host threads) race; a call returns after its stream has
+ * finished. */
+int mmc_energy_noalloc(mmc_ctx *ctx, int inst, double *E_total,
+                       double e_terms[5], double m_avg[3]);
+
 /* Energy density [J/m^3] of every real cell (N doubles), the selected terms
  * summed.  Replaces calEnergyKernel's per-cell output d_energy
  * (src/effective_H.h:35-43), which the reference dumps as E_density.ovf

@@ -109,6 +109,7 @@ EXPORTS = (
     "mmc_dims", "mmc_count", "mmc_build_tensor", "mmc_set_tensor",
     "mmc_get_tensor", "mmc_mirror_tensor", "mmc_set_m", "mmc_get_m",
     "mmc_init_random_m", "mmc_field", "mmc_energy_density", "mmc_trial_delta_e", "mmc_energy",
+    "mmc_energy_noalloc",
     "mmc_set_temperature",
     "mmc_sweeps", "mmc_autotune", "mmc_set_schedule", "mmc_sweep", "mmc_sync", "mmc_anneal", "mmc_set_option",
     "mmc_kernel_stats", "mmc_kernel_bytes",
@@ -172,6 +173,7 @@ def lib() -> C.CDLL:
     L.mmc_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
     L.mmc_set_schedule.argtypes = [vp, C.c_char_p]
     L.mmc_energy.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.mmc_energy_noalloc.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mmc_set_temperature.argtypes = [vp, C.c_int, C.c_double]
     L.mmc_sweeps.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32]
     L.mmc_sweep.argtypes = [vp, C.c_double, C.c_uint32, C.c_uint32]

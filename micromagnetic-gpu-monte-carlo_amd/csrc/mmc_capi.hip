@@ -16,6 +16,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <string>
 #include <sys/stat.h>
 #include <vector>
@@ -872,6 +873,38 @@ int alloc_all(mmc_ctx *c)
     return MMC_OK;
 }
 
+// ---- fixed scratch of mmc_energy_noalloc ------------------------------------
+// The energy reduction's scratch in a device array of the library instead of
+// memory a context allocates: the per-instance sums (8 doubles for each of
+// up to kNoAllocMaxInst instances) and then the block partials (16 doubles
+// per block of kThreads cells).  Each GPU holds its own copy until the
+// process ends; every call on a GPU shares it.
+constexpr size_t kScratchBytes = size_t(2) << 20;
+constexpr int kNoAllocMaxInst = 1024;
+constexpr size_t kScratchSums = size_t(8) * kNoAllocMaxInst * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices];   // device address of `scratch` per device
+std::mutex g_scratch_mu;
+
+// Look up the device address of `scratch` on the context's device (current
+// when the context is created), once per device.  In host code the array's
+// name is only a stand-in, never a device address.
+int scratch_setup(mmc_ctx *c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices)
+        return fail(c, MMC_EUNSUP, "device %d: at most %d devices", c->device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return MMC_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, MMC_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[c->device] = p;
+    return MMC_OK;
+}
+
 int create_common(const mmc_params *p, int count, int precision, mmc_ctx **out)
 {
     if (!out) return MMC_EINVAL;
@@ -886,6 +919,7 @@ int create_common(const mmc_params *p, int count, int precision, mmc_ctx **out)
     c->inst.resize(count);
     int rc;
     if (hipGetDevice(&c->device) != hipSuccess) { *out = c; return fail(c, MMC_EHIP, "no HIP device"); }
+    if ((rc = scratch_setup(c))) { *out = c; return rc; }
     {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess &&
@@ -1858,6 +1892,59 @@ int mmc_energy(mmc_ctx *c, int inst, double *E_total, double e_terms[5], double 
     const double *v = s.data() + 8 * inst;
     double tot = 0.0;
     double et[5];
+    for (int t = 0; t < 5; ++t) {
+        et[t] = v[t] * in.e_unit;
+        tot += et[t];
+    }
+    if (E_total) *E_total = tot;
+    if (e_terms) for (int t = 0; t < 5; ++t) e_terms[t] = et[t];
+    if (m_avg) for (int k = 0; k < 3; ++k) m_avg[k] = v[5 + k] / (double)in.N;
+    return MMC_OK;
+}
+
+int mmc_energy_noalloc(mmc_ctx *c, int inst, double *E_total, double e_terms[5], double m_avg[3])
+{
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    void *sp = c->device >= 0 && c->device < kMaxDevices ? g_scratch_ptr[c->device] : nullptr;
+    if (!sp) return fail(c, MMC_ESTATE, "scratch of device %d not set up", c->device);
+    if (c->count > kNoAllocMaxInst)
+        return fail(c, MMC_EINVAL, "mmc_energy_noalloc: %d instances, at most %d", c->count,
+                    kNoAllocMaxInst);
+    const size_t need = kScratchSums + (size_t)16 * c->cl.n() * sizeof(double);
+    if (need > kScratchBytes)
+        return fail(c, MMC_EINVAL,
+                    "mmc_energy_noalloc: %d reduction blocks need %zu bytes of scratch, above the "
+                    "%zu-byte maximum (at most %zu blocks of %d cells)", c->cl.n(), need,
+                    kScratchBytes, (kScratchBytes - kScratchSums) / (16 * sizeof(double)), kThreads);
+    double *sums = (double *)sp;
+    double *part = (double *)((unsigned char *)sp + kScratchSums);
+    // untimed launches on the context's own stream only: no events, no side
+    // streams, no device memory beyond what the context already holds
+    const bool timing = c->timing, conc = c->concurrent;
+    c->timing = false;
+    c->concurrent = false;
+    rc = c->tsize == 4 ? demag_chain<float>(c) : demag_chain<double>(c);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        hipStream_t st = c->stream;
+        e = c->tsize == 4
+                ? launch_energy<float>(st, c->dInst, c->cl.dev, c->cl.n(), (const float *)c->d_m,
+                                       (const float *)c->d_h, part, c->any_demag ? 1 : 0)
+                : launch_energy<double>(st, c->dInst, c->cl.dev, c->cl.n(), (const double *)c->d_m,
+                                        (const double *)c->d_h, part, c->any_demag ? 1 : 0);
+        if (e == hipSuccess) e = launch_energy_final(st, c->d_efirst, c->count, part, sums);
+    }
+    double v[8];
+    if (!rc && e == hipSuccess)
+        e = hipMemcpyAsync(v, sums + 8 * inst, sizeof(v), hipMemcpyDeviceToHost, c->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    c->timing = timing;
+    c->concurrent = conc;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(c, MMC_EHIP, "mmc_energy_noalloc: %s", hipGetErrorString(e));
+    const Inst &in = c->inst[inst];
+    double tot = 0.0, et[5];
     for (int t = 0; t < 5; ++t) {
         et[t] = v[t] * in.e_unit;
         tot += et[t];
