@@ -18,6 +18,8 @@ namespace mmc {
 template <int... Rs> struct Radices {};
 
 template <int N> struct FixedPlan;   // only the sizes listed are specialised
+template <> struct FixedPlan<16> { using type = Radices<4, 4>; };
+template <> struct FixedPlan<32> { using type = Radices<4, 4, 2>; };
 template <> struct FixedPlan<64> { using type = Radices<4, 4, 4>; };
 template <> struct FixedPlan<96> { using type = Radices<4, 4, 2, 3>; };
 template <> struct FixedPlan<128> { using type = Radices<4, 4, 4, 2>; };
@@ -32,6 +34,7 @@ template <> struct FixedPlan<2048> { using type = Radices<4, 4, 4, 4, 4, 2>; };
 // Same radix order as the host planner (make_plan: 4s first, then 2, 3, 5, 7).
 __host__ __device__ constexpr bool fixed_fft_size(int n)
 {
+    // y and x lengths; the z pass takes 16, 32 and 64 (fixed_z_size)
     return n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 384 ||
            n == 512 || n == 768 || n == 1024 || n == 2048;
 }
@@ -49,8 +52,13 @@ __device__ __forceinline__ cpx<T> twiddle_q(const cpx<T> *__restrict__ tw, int t
     return q == 0 ? w : (q == 1 ? r1 : r2);
 }
 
-// QTW: tw is the quarter table (twiddle_q) instead of the full one.
-template <typename T, int N, int ROWS, int NS, int R, int NT, bool QTW = false>
+// QTW: tw is the quarter table (twiddle_q) instead of the full one.  Element
+// i of row `row` sits at s[row*RS + i*ES]: contiguous rows (ES = 1, RS = N),
+// or rows side by side (ES > 1, e.g. the y columns of a z transform, RS =
+// 1), whose butterflies go to neighbouring lanes row by row so that their
+// LDS accesses are adjacent.
+template <typename T, int N, int ROWS, int NS, int R, int NT, bool QTW = false, int ES = 1,
+          int RS = N>
 __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
     constexpr int NB = N / R;
@@ -62,9 +70,9 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
     for (int b = 0; b < PER; ++b) {
         const int t = threadIdx.x + b * NT;
         if ((b + 1) * NT <= TOTAL || t < TOTAL) {
-            const int row = t / NB, j = t % NB;
+            const int row = ES == 1 ? t / NB : t % ROWS, j = ES == 1 ? t % NB : t / ROWS;
             #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = s[row * N + j + r * NB];
+            for (int r = 0; r < R; ++r) v[b][r] = s[row * RS + (j + r * NB) * ES];
         }
     }
     __syncthreads();
@@ -72,7 +80,7 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
     for (int b = 0; b < PER; ++b) {
         const int t = threadIdx.x + b * NT;
         if ((b + 1) * NT <= TOTAL || t < TOTAL) {
-            const int row = t / NB, j = t % NB;
+            const int row = ES == 1 ? t / NB : t % ROWS, j = ES == 1 ? t % NB : t / ROWS;
             const int k = j % NS;
             if (NS > 1) {
                 #pragma unroll
@@ -82,23 +90,24 @@ __device__ __forceinline__ void fixed_stage(cpx<T> *s, const cpx<T> *__restrict_
             Dft<T, R>::run(v[b]);
             const int o = (j - k) * R + k;
             #pragma unroll
-            for (int r = 0; r < R; ++r) s[row * N + o + r * NS] = v[b][r];
+            for (int r = 0; r < R; ++r) s[row * RS + (o + r * NS) * ES] = v[b][r];
         }
     }
     __syncthreads();
 }
 
-template <typename T, int N, int ROWS, int NS, int NT, bool QTW, typename RL> struct FixedStages;
-template <typename T, int N, int ROWS, int NS, int NT, bool QTW>
-struct FixedStages<T, N, ROWS, NS, NT, QTW, Radices<>> {
+template <typename T, int N, int ROWS, int NS, int NT, bool QTW, int ES, int RS, typename RL>
+struct FixedStages;
+template <typename T, int N, int ROWS, int NS, int NT, bool QTW, int ES, int RS>
+struct FixedStages<T, N, ROWS, NS, NT, QTW, ES, RS, Radices<>> {
     static __device__ __forceinline__ void run(cpx<T> *, const cpx<T> *) {}
 };
-template <typename T, int N, int ROWS, int NS, int NT, bool QTW, int R, int... Rest>
-struct FixedStages<T, N, ROWS, NS, NT, QTW, Radices<R, Rest...>> {
+template <typename T, int N, int ROWS, int NS, int NT, bool QTW, int ES, int RS, int R, int... Rest>
+struct FixedStages<T, N, ROWS, NS, NT, QTW, ES, RS, Radices<R, Rest...>> {
     static __device__ __forceinline__ void run(cpx<T> *s, const cpx<T> *tw)
     {
-        fixed_stage<T, N, ROWS, NS, R, NT, QTW>(s, tw);
-        FixedStages<T, N, ROWS, NS * R, NT, QTW, Radices<Rest...>>::run(s, tw);
+        fixed_stage<T, N, ROWS, NS, R, NT, QTW, ES, RS>(s, tw);
+        FixedStages<T, N, ROWS, NS * R, NT, QTW, ES, RS, Radices<Rest...>>::run(s, tw);
     }
 };
 
@@ -109,8 +118,21 @@ struct FixedStages<T, N, ROWS, NS, NT, QTW, Radices<R, Rest...>> {
 template <typename T, int N, int ROWS, int NT = kThreads, bool QTW = false>
 __device__ __forceinline__ void fft_fixed(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
-    FixedStages<T, N, ROWS, 1, NT, QTW, typename FixedPlan<N>::type>::run(s, tw);
+    FixedStages<T, N, ROWS, 1, NT, QTW, 1, N, typename FixedPlan<N>::type>::run(s, tw);
 }
+
+// Forward transform along the slow axis of a [N][ROWS] tile (element i of
+// row `row` at s[i*ROWS + row]): ROWS rows side by side, e.g. the y columns
+// of a z transform.  Same arithmetic as fft_fixed; synchronisation alike.
+template <typename T, int N, int ROWS, int NT = kThreads>
+__device__ __forceinline__ void fft_fixed_cols(cpx<T> *s, const cpx<T> *__restrict__ tw)
+{
+    FixedStages<T, N, ROWS, 1, NT, false, ROWS, 1, typename FixedPlan<N>::type>::run(s, tw);
+}
+
+// z lengths with a compile-time-planned z pass (3-D films of 8-32 layers,
+// or periodic z of 16, 32, 64 layers).
+__host__ __device__ constexpr bool fixed_z_size(int n) { return n == 16 || n == 32 || n == 64; }
 
 // Columns per y-pass workgroup of the fixed path: about 3072 complex per tile.
 __host__ __device__ constexpr int fixed_cols(int n) { return n >= 1024 ? 1 : 1024 / n; }

@@ -101,6 +101,91 @@ __global__ __launch_bounds__(kThreads) void k_zinv(const Inst *__restrict__ I,
     }
 }
 
+// Compile-time-planned z transforms (Pz = PZ, a fixed_z_size, magnetisation
+// only): item (inst, c, kx, y0) covers TY = z_cols y columns; the tile is [PZ][TY]
+// in LDS (z outer, y contiguous), loaded and stored with y along the lanes,
+// transformed along z with neighbouring lanes on neighbouring columns.
+// Same arithmetic as the runtime-plan path (same radix order and twiddles).
+template <typename T, int PZ>
+__global__ __launch_bounds__(kThreads) void k_zfwd_n(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     const cpx<T> *__restrict__ srcb,
+                                                     cpx<T> *__restrict__ dstb,
+                                                     const cpx<T> *__restrict__ twb)
+{
+    constexpr int TY = z_cols(sizeof(T));
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + PZ * TY;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, kx = w.b, y0 = w.c;
+    const int H = in.Hx, Ly = in.ny, Lz = in.nz;
+    const cpx<T> *src = srcb + in.a_off + ((long)c * H + kx) * Lz * Ly + y0;
+    cpx<T> *dst = dstb + in.b_off + ((long)c * H + kx) * PZ * Ly + y0;
+    const int nyt = min(TY, Ly - y0);
+    const auto twr = load_twiddles<T, PZ>(twb + in.twz);
+    staged_copy<PZ * TY, kThreads, cpx<T>>(
+        PZ * TY,
+        [&](int e) {
+            const int z = e / TY, y = e - z * TY;
+            T vx = T(0), vy = T(0);
+            if (z < Lz && y < nyt) {
+                const cpx<T> t = src[(long)z * Ly + y];
+                vx = t.x;
+                vy = t.y;
+            }
+            return cpx<T>{vx, vy};
+        },
+        [&](int e, cpx<T> v) { s[e] = v; });
+    store_twiddles<T, PZ>(tw, twr);
+    __syncthreads();
+    fft_fixed_cols<T, PZ, TY>(s, tw);
+    for (int e = threadIdx.x; e < PZ * TY; e += kThreads) {
+        const int z = e / TY, y = e - z * TY;
+        if (y < nyt) dst[(long)z * Ly + y] = s[e];
+    }
+}
+
+template <typename T, int PZ>
+__global__ __launch_bounds__(kThreads) void k_zinv_n(const Inst *__restrict__ I,
+                                                     const Work *__restrict__ W,
+                                                     const cpx<T> *__restrict__ srcb,
+                                                     cpx<T> *__restrict__ dstb,
+                                                     const cpx<T> *__restrict__ twb)
+{
+    constexpr int TY = z_cols(sizeof(T));
+    cpx<T> *s = reinterpret_cast<cpx<T> *>(g_smem);
+    cpx<T> *tw = s + PZ * TY;
+    const Work w = W[blockIdx.x];
+    const Inst &in = I[w.inst];
+    const int c = w.a, kx = w.b, y0 = w.c;
+    const int H = in.Hx, Ly = in.ny, Lz = in.nz;
+    const cpx<T> *src = srcb + in.b_off + ((long)c * H + kx) * PZ * Ly + y0;
+    cpx<T> *dst = dstb + in.a_off + ((long)c * H + kx) * Lz * Ly + y0;
+    const int nyt = min(TY, Ly - y0);
+    const auto twr = load_twiddles<T, PZ>(twb + in.twz);
+    staged_copy<PZ * TY, kThreads, cpx<T>>(
+        PZ * TY,
+        [&](int e) {
+            const int z = e / TY, y = e - z * TY;
+            T vx = T(0), vy = T(0);
+            if (y < nyt) {
+                const cpx<T> t = src[(long)z * Ly + y];
+                vx = t.x;
+                vy = t.y;
+            }
+            return cpx<T>{vx, vy};
+        },
+        [&](int e, cpx<T> v) { s[e] = cconj(v); });
+    store_twiddles<T, PZ>(tw, twr);
+    __syncthreads();
+    fft_fixed_cols<T, PZ, TY>(s, tw);
+    for (int e = threadIdx.x; e < Lz * TY; e += kThreads) {
+        const int z = e / TY, y = e - z * TY;
+        if (y < nyt) dst[(long)z * Ly + y] = cconj(s[e]);
+    }
+}
+
 // Khat component c6 at k point ki of instance `in`.  The instance's Khat
 // slot holds six planes of S = Hx*Pz*Py complex values; a plane stored real
 // (kmode 1) uses its first S scalars, a zero plane (kmode 0) is not read.
@@ -478,18 +563,32 @@ __global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict
 
 template <typename T>
 hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode,
-                       const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
+                       int fixed_pz, const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
+    if (mode == 0 && fixed_z_size(fixed_pz)) {
+        const dim3 g(n), b(kThreads);
+        if (fixed_pz == 16) MMC_LAUNCH((k_zfwd_n<T, 16>), g, b, lds, st, I, W, src, dst, tw);
+        else if (fixed_pz == 32) MMC_LAUNCH((k_zfwd_n<T, 32>), g, b, lds, st, I, W, src, dst, tw);
+        else MMC_LAUNCH((k_zfwd_n<T, 64>), g, b, lds, st, I, W, src, dst, tw);
+        return post();
+    }
     MMC_LAUNCH(k_zfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, src, dst, tw);
     return post();
 }
 
 template <typename T>
-hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, const cpx<T> *src,
-                       cpx<T> *dst, const cpx<T> *tw, size_t lds)
+hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_pz,
+                       const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
+    if (fixed_z_size(fixed_pz)) {
+        const dim3 g(n), b(kThreads);
+        if (fixed_pz == 16) MMC_LAUNCH((k_zinv_n<T, 16>), g, b, lds, st, I, W, src, dst, tw);
+        else if (fixed_pz == 32) MMC_LAUNCH((k_zinv_n<T, 32>), g, b, lds, st, I, W, src, dst, tw);
+        else MMC_LAUNCH((k_zinv_n<T, 64>), g, b, lds, st, I, W, src, dst, tw);
+        return post();
+    }
     MMC_LAUNCH(k_zinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, src, dst, tw);
     return post();
 }
@@ -557,6 +656,10 @@ hipError_t set_lds_ypass(int bytes)
     hipError_t e = hipSuccess;
     set_lds_one(e, (const void *)k_zfwd<T>, bytes);
     set_lds_one(e, (const void *)k_zinv<T>, bytes);
+#define MMC_ZL(PZ) set_lds_one(e, (const void *)k_zfwd_n<T, PZ>, bytes); \
+    set_lds_one(e, (const void *)k_zinv_n<T, PZ>, bytes);
+    MMC_ZL(16) MMC_ZL(32) MMC_ZL(64)
+#undef MMC_ZL
     set_lds_one(e, (const void *)k_ymul<T>, bytes);
 #define MMC_YL(NN) set_lds_one(e, (const void *)k_ymul_n<T, NN, 1>, bytes); \
     set_lds_one(e, (const void *)k_ymul_n<T, NN, fixed_cols(NN)>, bytes);
@@ -578,9 +681,9 @@ hipError_t set_lds_ypass(int bytes)
 }
 
 #define MMC_INST(T)                                                                           \
-    template hipError_t launch_zfwd<T>(hipStream_t, const Inst *, const Work *, int, int,     \
+    template hipError_t launch_zfwd<T>(hipStream_t, const Inst *, const Work *, int, int, int, \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
-    template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int,          \
+    template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int, int,     \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                        cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \

@@ -133,6 +133,8 @@ struct mmc_ctx {
     int any_pol = -1;                 // >= 0: the y pass and both fused passes are one
                                       // launch each over every instance (tile policy)
     bool any_ok = false;              // every instance admits those launches
+    int z_fixed = 0;                  // common compile-time-planned z length of the
+                                      // z passes (0: runtime plan)
     bool concurrent = false;          // per-length launches of a pass on side streams
     std::vector<hipStream_t> side;    // side streams (default priority) and the
     hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
@@ -532,10 +534,21 @@ void build_lists(mmc_ctx *c)
     for (List *l : {&c->xl, &c->zl, &c->yl, &c->cl, &c->ml, &c->mtl, &c->tl, &c->txl, &c->tzl, &c->tyl, &c->fl, &c->zls, &c->yzl, &c->rl}) reset(*l);
     std::map<std::pair<int, int>, Segment> zdclass;  // (y length, layers) -> items
     std::map<std::pair<int, int>, std::vector<Work>> zditems;
+    // z passes take the compile-time plan when every instance that has them
+    // shares one planned length
+    c->z_fixed = 0;
+    for (const Inst &in : c->inst) {
+        if (!in.cal_demag || in.Pz <= kZDirect || in.zd) continue;
+        if (c->z_fixed == 0 && fixed_z_size(in.Pz)) c->z_fixed = in.Pz;
+        else if (c->z_fixed != in.Pz) { c->z_fixed = -1; break; }
+    }
+    for (const Inst &in : c->inst)
+        if (in.cal_demag && in.Pz > 1 && in.Pz <= kZDirect && !in.zd) c->z_fixed = -1;
+    if (c->z_fixed < 0) c->z_fixed = 0;
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
         const bool zdirect = in.Pz <= kZDirect;
-        const int zstep = zdirect ? kThreads : in.tyz;
+        const int zstep = zdirect ? kThreads : (c->z_fixed ? z_cols(c->tsize) : in.tyz);
         const int zstep64 = zdirect ? kThreads : in.tyz64;
         if (in.cal_demag) {
             const bool xf = xfixed(c, in);
@@ -566,7 +579,9 @@ void build_lists(mmc_ctx *c)
                             c->zl.host.push_back({i, comp, kx, y0});
                             if (!in.zf) c->zls.host.push_back({i, comp, kx, y0});
                         }
-                if (!zdirect) c->zl.lds = std::max(c->zl.lds, (2 * (size_t)in.tyz + 1) * in.Pz * cs);
+                if (!zdirect)
+                    c->zl.lds = std::max(c->zl.lds, c->z_fixed ? ((size_t)z_cols(c->tsize) + 1) * in.Pz * cs
+                                                               : (2 * (size_t)in.tyz + 1) * in.Pz * cs);
             }
             const int fn = fixed_fft_size(in.Py) ? in.Py : 0;
             const std::pair<int, int> yk(fn, fn ? in.ncol : 0);
@@ -1067,7 +1082,7 @@ int pass_zfwd(mmc_ctx *c, bool sweep)
     const Bufs<T> b(c);
     const List &l = sweep ? c->zls : c->zl;
     if (l.n())
-        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(c->stream, c->dInst, l.dev, l.n(), 0, b.A, b.B, b.tw, l.lds); },
+        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(c->stream, c->dInst, l.dev, l.n(), 0, c->z_fixed, b.A, b.B, b.tw, l.lds); },
                         zbytes(c, sweep)));
     return MMC_OK;
 }
@@ -1095,7 +1110,7 @@ int pass_zyz(mmc_ctx *c, bool sweep = false)
     });
     if (rc) return rc;
     if (l.n())
-        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, l.dev, l.n(), b.B, b.A, b.tw, l.lds); },
+        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, l.dev, l.n(), c->z_fixed, b.B, b.A, b.tw, l.lds); },
                         zbytes(c, sweep)));
     return MMC_OK;
 }
@@ -1336,7 +1351,7 @@ int tensor_fft(mmc_ctx *c)
     int rc = MMC_OK;
     e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, 0, 0, c->d_K, KA, tw, c->txl.lds);
     if (e == hipSuccess && c->tzl.n())
-        e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, KA, KA, tw, c->tzl.lds);
+        e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, 0, KA, KA, tw, c->tzl.lds);
     if (e == hipSuccess)
         e = launch_yfwd_tensor<double>(st, c->dInst, c->tyl.dev, c->tyl.n(), KA, KH, tw, c->tyl.lds);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
