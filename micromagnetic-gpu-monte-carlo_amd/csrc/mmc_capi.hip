@@ -639,8 +639,11 @@ void build_lists(mmc_ctx *c)
                 const int rows = 2 * (in.tyf / in.nz);
                 for (int y0 = 0; y0 < in.ny; y0 += rows) fitems[fk].push_back({i, 0, 0, y0});
             } else {
-                for (int z = 0; z < in.nz; ++z)
-                    for (int y0 = 0; y0 < in.ny; y0 += 2 * in.tyf) fitems[fk].push_back({i, 0, z, y0});
+                // the layers of one row pair next to each other: a cell's z
+                // neighbours sit in the other layers' tiles, which then run on
+                // the same XCD (xcd_group below) and find those rows in its L2
+                for (int y0 = 0; y0 < in.ny; y0 += 2 * in.tyf)
+                    for (int z = 0; z < in.nz; ++z) fitems[fk].push_back({i, 0, z, y0});
             }
             Segment &fs = fclass[fk];
             fs.fixed_n = in.Px;
@@ -654,7 +657,8 @@ void build_lists(mmc_ctx *c)
     c->zls.lds = c->zl.lds;
     for (auto &kv : fitems) {
         const Inst &i0 = c->inst[kv.second.front().inst];
-        xcd_group(kv.second, line_share(2 * kv.first.second / (i0.zf ? i0.nz : 1), cs));
+        xcd_group(kv.second, i0.zf ? line_share(2 * kv.first.second / i0.nz, cs)
+                                   : line_share(2 * kv.first.second, cs) * i0.nz);
         Segment sg = fclass[kv.first];
         sg.begin = c->fl.n();
         sg.count = (int)kv.second.size();
