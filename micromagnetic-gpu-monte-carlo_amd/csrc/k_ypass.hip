@@ -200,6 +200,29 @@ __device__ __forceinline__ cpx<T> khat_at(const T *__restrict__ kr, const Inst &
     return {T(0), T(0)};
 }
 
+// khat_at split in two for values loaded early and used late: khat_raw
+// issues one 8-byte (FP64: 16-byte) load from an address chosen by the
+// plane's storage (complex: the value; real: the aligned pair of scalars
+// holding it; zero: the plane's first element, the same for every lane), and
+// khat_val forms the value where it is used.  No load is selected or
+// branched around, so no load waits for another.
+template <typename T>
+__device__ __forceinline__ cpx<T> khat_raw(const T *__restrict__ kr, const Inst &in, int c6,
+                                           long S, long ki)
+{
+    const int md = in.kmode[c6];
+    const cpx<T> *p = reinterpret_cast<const cpx<T> *>(kr + 2 * (in.khat_off + c6 * S));
+    return p[md == 2 ? ki : (md == 1 ? (ki >> 1) : 0)];
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> khat_val(const cpx<T> &raw, const Inst &in, int c6, long ki)
+{
+    const int md = in.kmode[c6];
+    if (md == 2) return raw;
+    return {md == 1 ? ((ki & 1) ? raw.y : raw.x) : T(0), T(0)};
+}
+
 // Symmetric 3x3 complex product (src/demag_cu.cu:98-111) of one k point,
 // in place, conjugated for the inverse transform that follows.
 template <typename T>
@@ -238,36 +261,38 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     const T *kr = reinterpret_cast<const T *>(khatb);
     const long cstride = (long)H * Pz * Ly;
     const long kstride = (long)H * Pz * N;
+    // k points past the item's columns read the last column's values (a
+    // valid address; the product of such lanes is never stored), so the
+    // loads need no select, which would wait for each of them in turn
     auto load_k6 = [&](int q, cpx<T> *k6) {
         const int e = threadIdx.x + q * kThreads;
-        const int col = e / N, ky = e % N;
-        const bool ok = e < C * N && col < ncol;
+        const int col = min(e / N, ncol - 1), ky = e % N;
         const long ki = ((long)(kx0 + col) * Pz + kz) * N + ky;
         #pragma unroll
-        for (int c6 = 0; c6 < 6; ++c6)
-            k6[c6] = ok ? khat_at(kr, in, c6, kstride, ki) : cpx<T>{T(0), T(0)};
+        for (int c6 = 0; c6 < 6; ++c6) k6[c6] = khat_raw(kr, in, c6, kstride, ki);
     };
+    // loads in the order their data is needed (vector loads complete in
+    // issue order): twiddles and spectrum rows, then the tensor values,
+    // which may still be in flight during the forward transform
+    const auto twr = load_twiddles<T, N>(twb + in.twy);
+    const auto rows = staged_load<ROWS * N, kThreads, cpx<T>>(ROWS * N, [&](int e) {
+        const int r = e / N, y = e % N;
+        const int comp = r / C, col = r % C;
+        // component-wise select: a conditionally loaded struct is kept in scratch
+        T vx = T(0), vy = T(0);
+        if (col < ncol && y < Ly) {
+            const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
+            vx = t.x;
+            vy = t.y;
+        }
+        return cpx<T>{vx, vy};
+    });
     cpx<T> k6p[PF ? Q : 1][6];
     if (PF) {
         #pragma unroll
         for (int q = 0; q < (PF ? Q : 1); ++q) load_k6(q, k6p[q]);
     }
-    const auto twr = load_twiddles<T, N>(twb + in.twy);
-    staged_copy<ROWS * N, kThreads, cpx<T>>(
-        ROWS * N,
-        [&](int e) {
-            const int r = e / N, y = e % N;
-            const int comp = r / C, col = r % C;
-            // component-wise select: a conditionally loaded struct is kept in scratch
-            T vx = T(0), vy = T(0);
-            if (col < ncol && y < Ly) {
-                const cpx<T> t = buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y];
-                vx = t.x;
-                vy = t.y;
-            }
-            return cpx<T>{vx, vy};
-        },
-        [&](int e, cpx<T> v) { s[e] = v; });
+    staged_store<ROWS * N, kThreads, cpx<T>>(ROWS * N, rows, [&](int e, cpx<T> v) { s[e] = v; });
     store_twiddles<T, N>(tw, twr);
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
@@ -285,10 +310,10 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
             cpx<T> k6[6];
             if (PF) {
                 #pragma unroll
-                for (int c6 = 0; c6 < 6; ++c6) k6[c6] = k6p[PF ? q : 0][c6];
+                for (int c6 = 0; c6 < 6; ++c6) k6[c6] = khat_val(k6p[PF ? q : 0][c6], in, c6, ky);
             } else {
                 #pragma unroll
-                for (int c6 = 0; c6 < 6; ++c6) k6[c6] = k6l[PF ? 0 : q][c6];
+                for (int c6 = 0; c6 < 6; ++c6) k6[c6] = khat_val(k6l[PF ? 0 : q][c6], in, c6, ky);
             }
             kmul_conj(s + (0 * C + col) * N + ky, s + (1 * C + col) * N + ky,
                       s + (2 * C + col) * N + ky, k6);
@@ -381,15 +406,16 @@ void k_yzconv_n(const Inst *__restrict__ I,
     const long astride = (long)H * NZ * Ly;        // component stride in A
     const T *kr = reinterpret_cast<const T *>(khatb);
     const long kstride = (long)H * Pz * N;         // component stride in Khat
+    // lanes past the length read the last k point (a valid address; their
+    // products are never formed), so the loads need no select
     auto load_k = [&](int q, cpx<T> (&k6)[ND][6]) {
-        const int ky = threadIdx.x + q * NT;
+        const int ky = min(threadIdx.x + q * NT, N - 1);
         #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const int slot = (d - (NZ - 1) + Pz) % Pz;
             const long ki = ((long)kx * Pz + slot) * N + ky;
             #pragma unroll
-            for (int c6 = 0; c6 < 6; ++c6)
-                k6[d][c6] = ky < N ? khat_at(kr, in, c6, kstride, ki) : cpx<T>{T(0), T(0)};
+            for (int c6 = 0; c6 < 6; ++c6) k6[d][c6] = khat_raw(kr, in, c6, kstride, ki);
         }
     };
     const auto twr = load_twiddles<T, N / 4, NT>(twb + in.twy);
@@ -431,6 +457,10 @@ void k_yzconv_n(const Inst *__restrict__ I,
         } else {
             load_k(q, kq);
         }
+        #pragma unroll
+        for (int d = 0; d < ND; ++d)
+            #pragma unroll
+            for (int c6 = 0; c6 < 6; ++c6) kq[d][c6] = khat_val(kq[d][c6], in, c6, ky);
         cpx<T> mv[NZ][3];
         #pragma unroll
         for (int z = 0; z < NZ; ++z)
