@@ -22,16 +22,22 @@ template <> struct FixedPlan<16> { using type = Radices<4, 4>; };
 template <> struct FixedPlan<32> { using type = Radices<4, 4, 2>; };
 template <> struct FixedPlan<64> { using type = Radices<4, 4, 4>; };
 template <> struct FixedPlan<96> { using type = Radices<4, 4, 2, 3>; };
-template <> struct FixedPlan<128> { using type = Radices<4, 4, 4, 2>; };
-template <> struct FixedPlan<192> { using type = Radices<4, 4, 4, 3>; };
-template <> struct FixedPlan<256> { using type = Radices<4, 4, 4, 4>; };
-template <> struct FixedPlan<384> { using type = Radices<4, 4, 4, 2, 3>; };
-template <> struct FixedPlan<512> { using type = Radices<4, 4, 4, 4, 2>; };
+template <> struct FixedPlan<128> { using type = Radices<8, 4, 4>; };
+template <> struct FixedPlan<192> { using type = Radices<8, 8, 3>; };
+template <> struct FixedPlan<256> { using type = Radices<8, 8, 4>; };
+template <> struct FixedPlan<384> { using type = Radices<8, 8, 2, 3>; };
+template <> struct FixedPlan<512> { using type = Radices<8, 8, 8>; };
 template <> struct FixedPlan<768> { using type = Radices<4, 4, 4, 4, 3>; };
 template <> struct FixedPlan<1024> { using type = Radices<4, 4, 4, 4, 4>; };
 template <> struct FixedPlan<2048> { using type = Radices<4, 4, 4, 4, 4, 2>; };
 
-// Same radix order as the host planner (make_plan: 4s first, then 2, 3, 5, 7).
+// Lengths 128-512 take radix-8 stages (fewer LDS passes and barriers: 64
+// uneven films' y pass 42.4 -> 37.7 us; at 1024 the y pass with the direct z
+// convolution is slower with them, 35.4 -> 40 us, so 1024 and 2048 stay
+// radix-4).  The runtime-plan path (make_plan: 4s first, then 2, 3, 5, 7)
+// never transforms a length that has a compile-time plan, except the FP64
+// tensor, so within a context every transform of a given length has one
+// arithmetic whichever kernel runs it.
 __host__ __device__ constexpr bool fixed_fft_size(int n)
 {
     // y and x lengths; the z pass takes 16, 32 and 64 (fixed_z_size)
@@ -39,8 +45,8 @@ __host__ __device__ constexpr bool fixed_fft_size(int n)
            n == 512 || n == 768 || n == 1024 || n == 2048;
 }
 
-// Twiddle exp(-2 pi i t / N) from a quarter table tw[u] = exp(-2 pi i u / N),
-// u < N/4, rotated by -i per quarter (exact); stage indices stay below 3N/4.
+// Twiddle exp(-2 pi i t / N), t < N, from a quarter table tw[u] =
+// exp(-2 pi i u / N), u < N/4, rotated by -i per quarter (exact).
 template <typename T, int N>
 __device__ __forceinline__ cpx<T> twiddle_q(const cpx<T> *__restrict__ tw, int t)
 {
@@ -49,7 +55,8 @@ __device__ __forceinline__ cpx<T> twiddle_q(const cpx<T> *__restrict__ tw, int t
     const cpx<T> w = tw[u];
     const cpx<T> r1 = {w.y, -w.x};    // -i w
     const cpx<T> r2 = {-w.x, -w.y};   // -w
-    return q == 0 ? w : (q == 1 ? r1 : r2);
+    const cpx<T> r3 = {-w.y, w.x};    // i w
+    return q == 0 ? w : (q == 1 ? r1 : (q == 2 ? r2 : r3));
 }
 
 // QTW: tw is the quarter table (twiddle_q) instead of the full one.  Element

@@ -68,6 +68,30 @@ template <typename T> struct Dft<T, 4> {
     }
 };
 
+// 8 points as two 4-point DFTs of the even and odd inputs joined by the
+// twiddles exp(-2 pi i k/8) (exact rotations by -i, and (1 -+ i)/sqrt2).
+template <typename T> struct Dft<T, 8> {
+    static __device__ __forceinline__ void run(cpx<T> *v)
+    {
+        cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<T, 4>::run(e);
+        Dft<T, 4>::run(o);
+        const T c = T(0.70710678118654752440);   // sqrt(2)/2
+        const cpx<T> o1 = {(o[1].x + o[1].y) * c, (o[1].y - o[1].x) * c};    // o1 (1 - i)/sqrt2
+        const cpx<T> o2 = {o[2].y, -o[2].x};                                  // -i o2
+        const cpx<T> o3 = {(o[3].y - o[3].x) * c, -(o[3].x + o[3].y) * c};   // o3 (-1 - i)/sqrt2
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
 template <typename T> struct Dft<T, 5> {
     static __device__ __forceinline__ void run(cpx<T> *v)
     {
