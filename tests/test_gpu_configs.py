@@ -3,8 +3,8 @@
 Each workload runs the kernel variants its geometry selects in the library
 (transform lengths, tile shapes, z handling), so each is compared with the
 oracle at full size: the 256x256 film with 20 periodic images in FP64 under
-row sweeps, the 128x128x16 film (padded 256x256x32, z on the runtime-plan
-transform), one instance of every size of the batched workload, and the
+row sweeps, the 128x128x16 film (padded 256x256x32, z on the compile-time
+32-point plan), one instance of every size of the batched workload, and the
 1024x1024x2 film with 10 images.  FP64: fields, energies and every cell's
 trial dE within 1e-12, three sweeps equal to the oracle's chain within 1e-10.
 FP32 bulk: H_eff and dE within 2e-6 of their largest values, energies within
@@ -73,7 +73,7 @@ def test_film256_pbc20_fp64_row_sweeps(orc, mmc):
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_film128x16_open(orc, mmc, precision):
     """128x128x16 open film (padded 256x256x32): the z transforms of length 32
-    run on the runtime-plan LDS path.  The GPU tensor is also checked against
+    run on the compile-time plan ([32][64] tiles).  The GPU tensor is also checked against
     the oracle's build, entry by entry within the Newell rounding noise."""
     kw = dict(FILM3D, nx=128, ny=128, nz=16, seed=128)
     c = mmc.Context(mmc.make_params(**kw), precision)

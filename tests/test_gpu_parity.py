@@ -20,10 +20,10 @@ CASES = [
     ("odd_pbc", dict(nx=15, ny=12, nz=1, pbc_x=1)),
     ("box3d_open", dict(nx=12, ny=10, nz=4)),
     ("box3d_pbcz", dict(nx=10, ny=8, nz=3, pbc_z=2)),
-    ("ragged_3d", dict(nx=7, ny=9, nz=5, pbc_y=1)),
+    ("ragged_3d", dict(nx=7, ny=9, nz=5, pbc_y=1)),          # Pz = 9, runtime z plan
     ("two_layer_3d_zconv", dict(nx=32, ny=32, nz=2)),
-    ("deep_z_3d", dict(nx=16, ny=16, nz=16)),                # Pz = 32, runtime plan
-    ("deep_z_pbcz_3d", dict(nx=12, ny=10, nz=16, pbc_z=1)),  # Pz = 16
+    ("deep_z_3d", dict(nx=16, ny=16, nz=16)),                # Pz = 32, compile-time z plan
+    ("deep_z_pbcz_3d", dict(nx=12, ny=10, nz=16, pbc_z=1)),  # Pz = 16, compile-time z plan
 ]
 
 
