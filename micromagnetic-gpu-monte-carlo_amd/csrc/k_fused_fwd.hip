@@ -10,8 +10,9 @@ namespace mmc {
 // cells are updated from the compact h and written to global m and to LDS,
 // then the rows are transformed and their half spectrum stored as k_xfwd_n
 // does (zf = 0), or transformed along z in registers and stored to the
-// xz-spectrum B as k_zfwd does (zf = 1).
-template <typename T, int N, int RZ>
+// xz-spectrum B as k_zfwd does (zf = 1; compiled only into the ZF variants,
+// as in k_xinv_metro).
+template <typename T, int N, int RZ, bool ZF>
 __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
                                                          const Work w,
                                                          T *__restrict__ mb,
@@ -29,7 +30,8 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     const int z0 = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
-    const int nzl = in.zf ? nz : 1;
+    const bool zf = ZF && in.zf;
+    const int nzl = zf ? nz : 1;
     const int R = RZ / nzl;
     const int nyt = min(2 * R, ny - y0);
     const long NN = in.N;
@@ -50,7 +52,7 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     constexpr int PFC = prefetch_cells<T, QC>();
     constexpr int CH = PFC ? PFC : (QC < gather_chunk<T>() ? QC : gather_chunk<T>());
     auto cell_at = [&](int e, int &zl, int &yy, int &j) {
-        zl = e / plane;
+        zl = ZF ? e / plane : 0;   // one layer per tile unless z is folded in
         const int r = e - zl * plane;
         yy = r / hw;
         j = r - yy * hw;
@@ -135,7 +137,7 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     fft_fixed<T, N, 3 * RZ>(s, tw);
     const int npair = (nyt + 1) >> 1;
     const int per = H * npair;
-    if (!in.zf) {
+    if (!zf) {
         const long cstride = (long)H * nz * ny;
         cpx<T> *dst = specb + in.a_off;
         for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
@@ -189,7 +191,7 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     }
 }
 
-template <typename T, int N, int RZ>
+template <typename T, int N, int RZ, bool ZF>
 __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict__ I,
                                                          const Work *__restrict__ W,
                                                          T *__restrict__ mb,
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd(const Inst *__restrict_
                                                          uint32_t sweep_off,
                                                          const cpx<T> *__restrict__ twb)
 {
-    metro_xfwd_tile<T, N, RZ>(I, W[blockIdx.x], mb, hb, specb, beta, ctl, sweep_off, twb);
+    metro_xfwd_tile<T, N, RZ, ZF>(I, W[blockIdx.x], mb, hb, specb, beta, ctl, sweep_off, twb);
 }
 
 // One launch over items of every x length of a batch (any_x_len), tile
@@ -220,7 +222,8 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd_any(const Inst *__restr
     switch (I[w.inst].Px) {
 #define MMC_ANY(NN)                                                                              \
     case NN:                                                                                     \
-        metro_xfwd_tile<T, NN, any_tile(POL, NN)>(I, w, mb, hb, specb, beta, ctl, sweep_off, twb); \
+        metro_xfwd_tile<T, NN, any_tile(POL, NN), false>(I, w, mb, hb, specb, beta, ctl, sweep_off, \
+                                                         twb);                                   \
         break;
     MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
 #undef MMC_ANY
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(kThreads) void k_metro_xfwd_any(const Inst *__restr
 
 template <typename T>
 hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
-                             int rows, T *m, const T *h, cpx<T> *spec, const double *beta,
+                             int rows, bool zf, T *m, const T *h, cpx<T> *spec, const double *beta,
                              const uint32_t *ctl, uint32_t sweep_off, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
@@ -246,8 +249,12 @@ hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n
     }
 #define MMC_MX_CASE(TT, NN, RR)                                                                  \
     if (fixed_n == NN && rows == RR) {                                                           \
-        MMC_LAUNCH((k_metro_xfwd<TT, NN, RR>), g, b, lds, st, I, W, m, h, spec, beta, ctl,       \
-                   sweep_off, tw);                                                               \
+        if (zf)                                                                                  \
+            MMC_LAUNCH((k_metro_xfwd<TT, NN, RR, true>), g, b, lds, st, I, W, m, h, spec, beta,  \
+                       ctl, sweep_off, tw);                                                      \
+        else                                                                                     \
+            MMC_LAUNCH((k_metro_xfwd<TT, NN, RR, false>), g, b, lds, st, I, W, m, h, spec, beta, \
+                       ctl, sweep_off, tw);                                                      \
         return post();                                                                           \
     }
     MMC_FUSED(MMC_MX_CASE, T, F)
@@ -260,7 +267,8 @@ hipError_t set_lds_fused_fwd(int bytes)
 {
     hipError_t e = hipSuccess;
     constexpr int F = fused_rows_base<T>();
-#define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_metro_xfwd<TT, NN, RR>, bytes);
+#define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_metro_xfwd<TT, NN, RR, false>, bytes); \
+    set_lds_one(e, (const void *)k_metro_xfwd<TT, NN, RR, true>, bytes);
     MMC_FUSED(MMC_FL, T, F)
 #undef MMC_FL
     set_lds_one(e, (const void *)k_metro_xfwd_any<T, 0>, bytes);
@@ -272,7 +280,7 @@ hipError_t set_lds_fused_fwd(int bytes)
 
 #define MMC_INST(T)                                                                           \
     template hipError_t launch_metro_xfwd<T>(hipStream_t, const Inst *, const Work *, int, int, \
-                                             int, T *, const T *, cpx<T> *, const double *,  \
+                                             int, bool, T *, const T *, cpx<T> *, const double *, \
                                              const uint32_t *, uint32_t, const cpx<T> *,     \
                                              size_t);                                        \
     template hipError_t set_lds_fused_fwd<T>(int);

@@ -15,8 +15,10 @@ namespace mmc {
 // x + y + z even); the h of colour-1 cells is stored compactly for the
 // second pass: hc[comp][(z*ny + y)*hw + j], x = 2j + ((y + z + 1) & 1),
 // hw = ceil(nx/2), in the instance's h region.  Same arithmetic as
-// k_zinv + k_xinv_n + k_metro.
-template <typename T, int N, int RZ>
+// k_zinv + k_xinv_n + k_metro.  Tiles that fold z in (zf) are compiled only
+// into the ZF variants: their unrolled z transform's bounds otherwise cost
+// every tile of the common kernels scalar registers and spill code.
+template <typename T, int N, int RZ, bool ZF>
 __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
                                                          const Work w,
                                                          const cpx<T> *__restrict__ specb,
@@ -33,7 +35,8 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     const int z0 = w.b, y0 = w.c;
     constexpr int H = N / 2 + 1;
     const int nx = in.nx, ny = in.ny, nz = in.nz;
-    const int nzl = in.zf ? nz : 1;
+    const bool zf = ZF && in.zf;
+    const int nzl = zf ? nz : 1;
     const int R = RZ / nzl;
     const int nyt = min(2 * R, ny - y0);
     const int npair = (nyt + 1) >> 1;
@@ -54,7 +57,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     constexpr int PFC = 0;
     constexpr int CH = QC < gather_chunk<T>() ? QC : gather_chunk<T>();
     auto cell_at = [&](int e, int &zl, int &yy, int &j) {
-        zl = e / plane;
+        zl = ZF ? e / plane : 0;   // one layer per tile unless z is folded in
         const int r = e - zl * plane;
         yy = r / hw;
         j = r - yy * hw;
@@ -84,7 +87,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
         }
     }
     const int per = H * npair;
-    if (!in.zf) {
+    if (!zf) {
         const long cstride = (long)H * nz * ny;
         const cpx<T> *src = specb + in.a_off;
         staged_copy<3 * H * RZ, kThreads, Pair2<cpx<T>>>(
@@ -182,7 +185,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     }
 }
 
-template <typename T, int N, int RZ>
+template <typename T, int N, int RZ, bool ZF>
 __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict__ I,
                                                          const Work *__restrict__ W,
                                                          const cpx<T> *__restrict__ specb,
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro(const Inst *__restrict_
                                                          uint32_t sweep_off,
                                                          const cpx<T> *__restrict__ twb)
 {
-    xinv_metro_tile<T, N, RZ>(I, W[blockIdx.x], specb, mb, hb, beta, ctl, sweep_off, twb);
+    xinv_metro_tile<T, N, RZ, ZF>(I, W[blockIdx.x], specb, mb, hb, beta, ctl, sweep_off, twb);
 }
 
 // One launch over items of every x length of a batch (any_x_len), tile
@@ -211,7 +214,8 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro_any(const Inst *__restr
     switch (I[w.inst].Px) {
 #define MMC_ANY(NN)                                                                              \
     case NN:                                                                                     \
-        xinv_metro_tile<T, NN, any_tile(POL, NN)>(I, w, specb, mb, hb, beta, ctl, sweep_off, twb); \
+        xinv_metro_tile<T, NN, any_tile(POL, NN), false>(I, w, specb, mb, hb, beta, ctl, sweep_off, \
+                                                         twb);                                   \
         break;
     MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
 #undef MMC_ANY
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_xinv_metro_any(const Inst *__restr
 
 template <typename T>
 hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
-                             int rows, const cpx<T> *spec, T *m, T *h, const double *beta,
+                             int rows, bool zf, const cpx<T> *spec, T *m, T *h, const double *beta,
                              const uint32_t *ctl, uint32_t sweep_off, const cpx<T> *tw, size_t lds)
 {
     if (n <= 0) return hipSuccess;
@@ -237,8 +241,12 @@ hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n
     }
 #define MMC_XM_CASE(TT, NN, RR)                                                                  \
     if (fixed_n == NN && rows == RR) {                                                           \
-        MMC_LAUNCH((k_xinv_metro<TT, NN, RR>), g, b, lds, st, I, W, spec, m, h, beta, ctl,       \
-                   sweep_off, tw);                                                               \
+        if (zf)                                                                                  \
+            MMC_LAUNCH((k_xinv_metro<TT, NN, RR, true>), g, b, lds, st, I, W, spec, m, h, beta,  \
+                       ctl, sweep_off, tw);                                                      \
+        else                                                                                     \
+            MMC_LAUNCH((k_xinv_metro<TT, NN, RR, false>), g, b, lds, st, I, W, spec, m, h, beta, \
+                       ctl, sweep_off, tw);                                                      \
         return post();                                                                           \
     }
     MMC_FUSED(MMC_XM_CASE, T, F)
@@ -251,7 +259,8 @@ hipError_t set_lds_fused_inv(int bytes)
 {
     hipError_t e = hipSuccess;
     constexpr int F = fused_rows_base<T>();
-#define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_xinv_metro<TT, NN, RR>, bytes);
+#define MMC_FL(TT, NN, RR) set_lds_one(e, (const void *)k_xinv_metro<TT, NN, RR, false>, bytes); \
+    set_lds_one(e, (const void *)k_xinv_metro<TT, NN, RR, true>, bytes);
     MMC_FUSED(MMC_FL, T, F)
 #undef MMC_FL
     set_lds_one(e, (const void *)k_xinv_metro_any<T, 0>, bytes);
@@ -263,7 +272,7 @@ hipError_t set_lds_fused_inv(int bytes)
 
 #define MMC_INST(T)                                                                           \
     template hipError_t launch_xinv_metro<T>(hipStream_t, const Inst *, const Work *, int, int, \
-                                             int, const cpx<T> *, T *, T *, const double *,  \
+                                             int, bool, const cpx<T> *, T *, T *, const double *, \
                                              const uint32_t *, uint32_t, const cpx<T> *,     \
                                              size_t);                                        \
     template hipError_t set_lds_fused_inv<T>(int);

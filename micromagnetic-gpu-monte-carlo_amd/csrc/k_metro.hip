@@ -266,11 +266,11 @@ __device__ __forceinline__ void cell_energy(const Inst &in, const T *mb, const T
     T cnt;
     stencil(in, m, x, y, z, ex, dmi, cnt);
     const T hex_x = ex.x - cnt * mi.x, hex_y = ex.y - cnt * mi.y, hex_z = ex.z - cnt * mi.z;
-    const T mu = mi.x * T(in.u[0]) + mi.y * T(in.u[1]) + mi.z * T(in.u[2]);
+    const T mu = mi.x * Ph<T>::u(in, 0) + mi.y * Ph<T>::u(in, 1) + mi.z * Ph<T>::u(in, 2);
     e[0] = double(T(-0.5) * (mi.x * hex_x + mi.y * hex_y + mi.z * hex_z));
     e[1] = double(T(-0.5) * (mi.x * dmi.x + mi.y * dmi.y + mi.z * dmi.z));
-    e[2] = double(-T(in.kap) * mu * mu);
-    e[3] = double(-(mi.x * T(in.b[0]) + mi.y * T(in.b[1]) + mi.z * T(in.b[2])));
+    e[2] = double(-Ph<T>::kap(in) * mu * mu);
+    e[3] = double(-(mi.x * Ph<T>::b(in, 0) + mi.y * Ph<T>::b(in, 1) + mi.z * Ph<T>::b(in, 2)));
     e[4] = 0.0;
     if (use_demag) {
         const T *h = hb + in.h_off;
@@ -357,11 +357,11 @@ __global__ __launch_bounds__(kThreads) void k_field(const Inst *__restrict__ I,
     if (terms & 1) { hx += ex.x - cnt * mi.x; hy += ex.y - cnt * mi.y; hz += ex.z - cnt * mi.z; }
     if (terms & 2) { hx += dmi.x; hy += dmi.y; hz += dmi.z; }
     if (terms & 4) {
-        const T mu = mi.x * T(in.u[0]) + mi.y * T(in.u[1]) + mi.z * T(in.u[2]);
-        const T f = T(2) * T(in.kap) * mu;
-        hx += f * T(in.u[0]); hy += f * T(in.u[1]); hz += f * T(in.u[2]);
+        const T mu = mi.x * Ph<T>::u(in, 0) + mi.y * Ph<T>::u(in, 1) + mi.z * Ph<T>::u(in, 2);
+        const T f = T(2) * Ph<T>::kap(in) * mu;
+        hx += f * Ph<T>::u(in, 0); hy += f * Ph<T>::u(in, 1); hz += f * Ph<T>::u(in, 2);
     }
-    if (terms & 8) { hx += T(in.b[0]); hy += T(in.b[1]); hz += T(in.b[2]); }
+    if (terms & 8) { hx += Ph<T>::b(in, 0); hy += Ph<T>::b(in, 1); hz += Ph<T>::b(in, 2); }
     if ((terms & 16) && in.cal_demag) {
         const T *h = hb + in.h_off;
         hx += h[i]; hy += h[N + i]; hz += h[2 * N + i];

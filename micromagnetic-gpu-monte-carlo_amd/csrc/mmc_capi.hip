@@ -295,6 +295,17 @@ void choose_mtile(const mmc_ctx *c, Inst &in)
         }
 }
 
+// FP32 copies of an instance's physics constants (Inst::pf).
+void round_phys(Inst &in)
+{
+    for (int a = 0; a < 3; ++a) in.pf.lam[a] = (float)in.lam[a];
+    for (int a = 0; a < 2; ++a) in.pf.dmi[a] = (float)in.dmi[a];
+    in.pf.kap = (float)in.kap;
+    for (int a = 0; a < 3; ++a) in.pf.u[a] = (float)in.u[a];
+    for (int a = 0; a < 3; ++a) in.pf.b[a] = (float)in.b[a];
+    for (int k = 0; k < 6; ++k) in.pf.K0[k] = (float)in.K0[k];
+}
+
 int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
 {
     if (p.nx < 1 || p.ny < 1 || p.nz < 1)
@@ -359,6 +370,7 @@ int setup_inst(mmc_ctx *c, int idx, const mmc_params &p, Inst &in)
     in.e_unit = mu0 * ms * ms * p.dx * p.dy * p.dz;
     in.key[0] = (uint32_t)p.seed;
     in.key[1] = (uint32_t)(p.seed >> 32);
+    round_phys(in);
     return MMC_OK;
 }
 
@@ -1163,16 +1175,18 @@ int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
         int rc = pass_zyz<T>(c, true);
         if (rc) return rc;
         const Bufs<T> b(c);
+        bool zf = false;
+        for (const Inst &in : c->inst) zf = zf || in.zf;
         rc = run_segments(c, c->fl, K_XINVM, [&](const Segment &sg, hipStream_t s) {
             return launch_xinv_metro<T>(s, c->dInst, c->fl.dev + sg.begin, sg.count, sg.fixed_n,
-                                        sg.rows, b.A, b.m, b.h, c->d_beta, c->d_ctl, sweep_off, b.tw,
-                                        sg.lds);
+                                        sg.rows, zf, b.A, b.m, b.h, c->d_beta, c->d_ctl, sweep_off,
+                                        b.tw, sg.lds);
         });
         if (rc) return rc;
         return run_segments(c, c->fl, K_MXFWD, [&](const Segment &sg, hipStream_t s) {
             return launch_metro_xfwd<T>(s, c->dInst, c->fl.dev + sg.begin, sg.count, sg.fixed_n,
-                                        sg.rows, b.m, b.h, b.A, c->d_beta, c->d_ctl, sweep_off, b.tw,
-                                        sg.lds);
+                                        sg.rows, zf, b.m, b.h, b.A, c->d_beta, c->d_ctl, sweep_off,
+                                        b.tw, sg.lds);
         });
     }
     int rc = demag_chain<T>(c);
@@ -1406,6 +1420,7 @@ int refresh_mirror(mmc_ctx *c)
         for (int k = 0; k < 6; ++k)
             HIPCHK(c, hipMemcpy(&in.K0[k], c->d_K + in.k_off + k * in.NP, sizeof(double),
                                 hipMemcpyDeviceToHost));
+        round_phys(in);
     }
     HIPCHK(c, hipMemcpy(c->dInst, c->inst.data(), sizeof(Inst) * c->count, hipMemcpyHostToDevice));
     return MMC_OK;

@@ -235,6 +235,25 @@ __device__ __forceinline__ V3<T> load_m(const T *m, uint32_t N, uint32_t i)
     return {m[i], m[N + i], m[2u * N + i]};
 }
 
+// Physics constants of an instance in the working precision T: the FP64
+// fields, or for FP32 their copies rounded once on the host (Inst::pf).
+template <typename T> struct Ph {
+    static __device__ __forceinline__ T lam(const Inst &in, int a) { return in.lam[a]; }
+    static __device__ __forceinline__ T dmi(const Inst &in, int a) { return in.dmi[a]; }
+    static __device__ __forceinline__ T kap(const Inst &in) { return in.kap; }
+    static __device__ __forceinline__ T u(const Inst &in, int a) { return in.u[a]; }
+    static __device__ __forceinline__ T b(const Inst &in, int a) { return in.b[a]; }
+    static __device__ __forceinline__ T K0(const Inst &in, int k) { return in.K0[k]; }
+};
+template <> struct Ph<float> {
+    static __device__ __forceinline__ float lam(const Inst &in, int a) { return in.pf.lam[a]; }
+    static __device__ __forceinline__ float dmi(const Inst &in, int a) { return in.pf.dmi[a]; }
+    static __device__ __forceinline__ float kap(const Inst &in) { return in.pf.kap; }
+    static __device__ __forceinline__ float u(const Inst &in, int a) { return in.pf.u[a]; }
+    static __device__ __forceinline__ float b(const Inst &in, int a) { return in.pf.b[a]; }
+    static __device__ __forceinline__ float K0(const Inst &in, int k) { return in.pf.K0[k]; }
+};
+
 // Bond sums of one cell from its six neighbour values (a missing bond
 // passes zeros), reduced units (field per Ms):
 //   ex  = sum_a lam_a * sum_{j in nbrs_a} m_j      (exchange, neighbours only)
@@ -244,23 +263,24 @@ __device__ __forceinline__ void bond_sums(const Inst &in, const V3<T> &xp, const
                                           const V3<T> &yp, const V3<T> &ym, const V3<T> &zp,
                                           const V3<T> &zm, V3<T> &ex, V3<T> &dmi)
 {
-    ex = {T(0), T(0), T(0)};
-    dmi = {T(0), T(0), T(0)};
     {   // x bonds
-        const T lam = T(in.lam[0]), dm = T(in.dmi[0]);
-        ex.x += lam * (xp.x + xm.x); ex.y += lam * (xp.y + xm.y); ex.z += lam * (xp.z + xm.z);
-        dmi.x += dm * (xp.z - xm.z);
-        dmi.z -= dm * (xp.x - xm.x);
+        const T lam = Ph<T>::lam(in, 0), dm = Ph<T>::dmi(in, 0);
+        ex = {lam * (xp.x + xm.x), lam * (xp.y + xm.y), lam * (xp.z + xm.z)};
+        dmi = {dm * (xp.z - xm.z), T(0), -(dm * (xp.x - xm.x))};
     }
     {   // y bonds
-        const T lam = T(in.lam[1]), dm = T(in.dmi[1]);
-        ex.x += lam * (yp.x + ym.x); ex.y += lam * (yp.y + ym.y); ex.z += lam * (yp.z + ym.z);
-        dmi.y += dm * (yp.z - ym.z);
-        dmi.z -= dm * (yp.y - ym.y);
+        const T lam = Ph<T>::lam(in, 1), dm = Ph<T>::dmi(in, 1);
+        ex.x = fmad(lam, yp.x + ym.x, ex.x);
+        ex.y = fmad(lam, yp.y + ym.y, ex.y);
+        ex.z = fmad(lam, yp.z + ym.z, ex.z);
+        dmi.y = dm * (yp.z - ym.z);
+        dmi.z = fmad(-dm, yp.y - ym.y, dmi.z);
     }
     if (in.nz > 1) {   // z bonds: exchange only
-        const T lam = T(in.lam[2]);
-        ex.x += lam * (zp.x + zm.x); ex.y += lam * (zp.y + zm.y); ex.z += lam * (zp.z + zm.z);
+        const T lam = Ph<T>::lam(in, 2);
+        ex.x = fmad(lam, zp.x + zm.x, ex.x);
+        ex.y = fmad(lam, zp.y + zm.y, ex.y);
+        ex.z = fmad(lam, zp.z + zm.z, ex.z);
     }
 }
 
@@ -278,24 +298,26 @@ __device__ __forceinline__ void stencil(const Inst &in, const T *m, int x, int y
     const uint32_t unx = nx, uny = ny;
     const uint32_t row = cell_index(unx, uny, 0, y, z);
     int j;
-    if ((j = nbr(x, +1, nx, in.pbc[0])) >= 0) { xp = load_m(m, N, row + j); cnt += T(in.lam[0]); }
-    if ((j = nbr(x, -1, nx, in.pbc[0])) >= 0) { xm = load_m(m, N, row + j); cnt += T(in.lam[0]); }
-    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) { yp = load_m(m, N, cell_index(unx, uny, x, j, z)); cnt += T(in.lam[1]); }
-    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) { ym = load_m(m, N, cell_index(unx, uny, x, j, z)); cnt += T(in.lam[1]); }
+    if ((j = nbr(x, +1, nx, in.pbc[0])) >= 0) { xp = load_m(m, N, row + j); cnt += Ph<T>::lam(in, 0); }
+    if ((j = nbr(x, -1, nx, in.pbc[0])) >= 0) { xm = load_m(m, N, row + j); cnt += Ph<T>::lam(in, 0); }
+    if ((j = nbr(y, +1, ny, in.pbc[1])) >= 0) { yp = load_m(m, N, cell_index(unx, uny, x, j, z)); cnt += Ph<T>::lam(in, 1); }
+    if ((j = nbr(y, -1, ny, in.pbc[1])) >= 0) { ym = load_m(m, N, cell_index(unx, uny, x, j, z)); cnt += Ph<T>::lam(in, 1); }
     if (nz > 1) {
-        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) { zp = load_m(m, N, cell_index(unx, uny, x, y, j)); cnt += T(in.lam[2]); }
-        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) { zm = load_m(m, N, cell_index(unx, uny, x, y, j)); cnt += T(in.lam[2]); }
+        if ((j = nbr(z, +1, nz, in.pbc[2])) >= 0) { zp = load_m(m, N, cell_index(unx, uny, x, y, j)); cnt += Ph<T>::lam(in, 2); }
+        if ((j = nbr(z, -1, nz, in.pbc[2])) >= 0) { zm = load_m(m, N, cell_index(unx, uny, x, y, j)); cnt += Ph<T>::lam(in, 2); }
     }
     bond_sums(in, xp, xm, yp, ym, zp, zm, ex, dmi);
 }
 
+// a.K0.a for the symmetric self tensor K0 = (xx, xy, xz, yy, yz, zz).
 template <typename T>
 __device__ __forceinline__ T quad_k0(const Inst &in, V3<T> a)
 {
-    const T k0 = T(in.K0[0]), k1 = T(in.K0[1]), k2 = T(in.K0[2]);
-    const T k3 = T(in.K0[3]), k4 = T(in.K0[4]), k5 = T(in.K0[5]);
-    return k0 * a.x * a.x + k3 * a.y * a.y + k5 * a.z * a.z +
-           T(2) * (k1 * a.x * a.y + k2 * a.x * a.z + k4 * a.y * a.z);
+    const T k0 = Ph<T>::K0(in, 0), k1 = Ph<T>::K0(in, 1), k2 = Ph<T>::K0(in, 2);
+    const T k3 = Ph<T>::K0(in, 3), k4 = Ph<T>::K0(in, 4), k5 = Ph<T>::K0(in, 5);
+    const T kx = fmad(k0, a.x, fmad(T(2) * k1, a.y, T(2) * k2 * a.z));
+    const T ky = fmad(k3, a.y, T(2) * k4 * a.z);
+    return fmad(a.x, kx, fmad(a.y, ky, k5 * a.z * a.z));
 }
 
 template <typename T> struct Move {
@@ -324,21 +346,24 @@ __device__ __forceinline__ T move_de(const Inst &in, const V3<T> &mi, const V3<T
                                      const V3<T> &dmi, const V3<T> &hi, int use_demag,
                                      const V3<T> &mn)
 {
-    V3<T> g = {ex.x + dmi.x + T(in.b[0]), ex.y + dmi.y + T(in.b[1]), ex.z + dmi.z + T(in.b[2])};
+    // with d = m' - m and the frozen demag field h (which includes the
+    // cell's own contribution K0 m):
+    //   -d.(h - K0 m) - [m'K0m' - mK0m]/2 = -d.h - d.K0.d/2
+    // (K0 symmetric), one quadratic form instead of three
+    V3<T> g = {ex.x + dmi.x + Ph<T>::b(in, 0), ex.y + dmi.y + Ph<T>::b(in, 1),
+               ex.z + dmi.z + Ph<T>::b(in, 2)};
     if (use_demag) {
-        const T k0 = T(in.K0[0]), k1 = T(in.K0[1]), k2 = T(in.K0[2]);
-        const T k3 = T(in.K0[3]), k4 = T(in.K0[4]), k5 = T(in.K0[5]);
-        g.x += hi.x - (k0 * mi.x + k1 * mi.y + k2 * mi.z);
-        g.y += hi.y - (k1 * mi.x + k3 * mi.y + k4 * mi.z);
-        g.z += hi.z - (k2 * mi.x + k4 * mi.y + k5 * mi.z);
+        g.x += hi.x;
+        g.y += hi.y;
+        g.z += hi.z;
     }
     const V3<T> d = {mn.x - mi.x, mn.y - mi.y, mn.z - mi.z};
-    T dE = -(d.x * g.x + d.y * g.y + d.z * g.z);
-    const T ux = T(in.u[0]), uy = T(in.u[1]), uz = T(in.u[2]);
-    const T pu = mn.x * ux + mn.y * uy + mn.z * uz;
-    const T ou = mi.x * ux + mi.y * uy + mi.z * uz;
-    dE -= T(in.kap) * (pu * pu - ou * ou);
-    if (use_demag) dE -= T(0.5) * (quad_k0(in, mn) - quad_k0(in, mi));
+    T dE = -fmad(d.x, g.x, fmad(d.y, g.y, d.z * g.z));
+    const T ux = Ph<T>::u(in, 0), uy = Ph<T>::u(in, 1), uz = Ph<T>::u(in, 2);
+    const T pu = fmad(mn.x, ux, fmad(mn.y, uy, mn.z * uz));
+    const T ou = fmad(mi.x, ux, fmad(mi.y, uy, mi.z * uz));
+    dE = fmad(-Ph<T>::kap(in), fmad(pu, pu, -(ou * ou)), dE);
+    if (use_demag) dE = fmad(T(-0.5), quad_k0(in, d), dE);
     return dE;
 }
 

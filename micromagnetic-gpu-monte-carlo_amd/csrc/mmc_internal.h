@@ -71,6 +71,12 @@ struct Inst {
     double u[3];          // unit anisotropy axis
     double b[3];          // B / (mu0 Ms)
     double K0[6];         // self ("mirror") tensor
+    // the constants above rounded to FP32 once on the host, so FP32 kernels
+    // read them as they are rather than converting an FP64 value at each use
+    // (the same rounding either way)
+    struct PhysF {
+        float lam[3], dmi[2], kap, u[3], b[3], K0[6];
+    } pf;
     double ms;            // Ms [A/m]
     double e_unit;        // mu0 Ms^2 V [J]
     uint32_t key[2];      // Philox key = seed
@@ -144,16 +150,17 @@ hipError_t launch_xinv(hipStream_t st, const Inst *I, const Work *W, int n,
 template <typename T> constexpr int fused_rows_base() { return sizeof(T) == 4 ? 2048 : 1024; }
 constexpr int fused_rows(int base, int n) { return base / n > 1 ? base / n : 1; }
 // Inverse x pass + colour 0: reads spec, updates m, stores colour-1 h
-// compactly in the h buffer.
+// compactly in the h buffer.  zf: some item's instance folds z into its
+// tiles (Inst::zf; the kernels without zf have no z-folding code).
 template <typename T>
 hipError_t launch_xinv_metro(hipStream_t st, const Inst *I, const Work *W, int n,
-                             int fixed_n, int rows, const cpx<T> *spec, T *m,
+                             int fixed_n, int rows, bool zf, const cpx<T> *spec, T *m,
                              T *h, const double *beta, const uint32_t *ctl,
                              uint32_t sweep_off, const cpx<T> *tw, size_t lds);
 // Colour 1 (compact h) + forward x pass of the updated rows into spec.
 template <typename T>
 hipError_t launch_metro_xfwd(hipStream_t st, const Inst *I, const Work *W, int n,
-                             int fixed_n, int rows, T *m, const T *h,
+                             int fixed_n, int rows, bool zf, T *m, const T *h,
                              cpx<T> *spec, const double *beta,
                              const uint32_t *ctl, uint32_t sweep_off,
                              const cpx<T> *tw, size_t lds);
