@@ -31,6 +31,34 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
 }
 template <typename T>
 __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+
+// acc + m0 k0 + m1 k1 + m2 k2 (complex): each part a chain of fused
+// multiply-adds (a row of the symmetric 3x3 tensor product in k-space).
+template <typename T>
+__device__ __forceinline__ cpx<T> cdot3_acc(cpx<T> acc, cpx<T> m0, cpx<T> k0, cpx<T> m1, cpx<T> k1,
+                                            cpx<T> m2, cpx<T> k2)
+{
+    T re = fmad(m0.x, k0.x, acc.x), im = fmad(m0.x, k0.y, acc.y);
+    re = fmad(-m0.y, k0.y, re); im = fmad(m0.y, k0.x, im);
+    re = fmad(m1.x, k1.x, re);  im = fmad(m1.x, k1.y, im);
+    re = fmad(-m1.y, k1.y, re); im = fmad(m1.y, k1.x, im);
+    re = fmad(m2.x, k2.x, re);  im = fmad(m2.x, k2.y, im);
+    re = fmad(-m2.y, k2.y, re); im = fmad(m2.y, k2.x, im);
+    return {re, im};
+}
+// m0 k0 + m1 k1 + m2 k2
+template <typename T>
+__device__ __forceinline__ cpx<T> cdot3(cpx<T> m0, cpx<T> k0, cpx<T> m1, cpx<T> k1, cpx<T> m2,
+                                        cpx<T> k2)
+{
+    T re = m0.x * k0.x, im = m0.x * k0.y;
+    re = fmad(-m0.y, k0.y, re); im = fmad(m0.y, k0.x, im);
+    re = fmad(m1.x, k1.x, re);  im = fmad(m1.x, k1.y, im);
+    re = fmad(-m1.y, k1.y, re); im = fmad(m1.y, k1.x, im);
+    re = fmad(m2.x, k2.x, re);  im = fmad(m2.x, k2.y, im);
+    re = fmad(-m2.y, k2.y, re); im = fmad(m2.y, k2.x, im);
+    return {re, im};
+}
 template <typename T>
 __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>

@@ -51,10 +51,11 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     constexpr int QC = (RZ * N + kThreads - 1) / kThreads;
     constexpr int PFC = prefetch_cells<T, QC>();
     constexpr int CH = PFC ? PFC : (QC < gather_chunk<T>() ? QC : gather_chunk<T>());
+    const Div dhw(hw);
     auto cell_at = [&](int e, int &zl, int &yy, int &j) {
         zl = ZF ? e / plane : 0;   // one layer per tile unless z is folded in
         const int r = e - zl * plane;
-        yy = r / hw;
+        yy = dhw(r);
         j = r - yy * hw;
     };
     V3<T> chi[CH];
@@ -137,12 +138,13 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     fft_fixed<T, N, 3 * RZ>(s, tw);
     const int npair = (nyt + 1) >> 1;
     const int per = H * npair;
+    const Div dper(per), dnp(npair);
     if (!zf) {
         const long cstride = (long)H * nz * ny;
         cpx<T> *dst = specb + in.a_off;
         for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
-            const int c = e / per, r = e - c * per;
-            const int k = r / npair, q = r - k * npair;
+            const int c = dper(e), r = e - c * per;
+            const int k = dnp(r), q = r - k * npair;
             const cpx<T> *row = s + (c * RZ + q) * N;
             const cpx<T> A = row[k];
             const cpx<T> B = row[k ? N - k : 0];
@@ -162,8 +164,8 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
         cpx<T> *dst = specb + in.b_off;
         const cpx<T> *twz = twb + in.twz;
         for (int e = threadIdx.x; e < 3 * per; e += kThreads) {
-            const int c = e / per, r = e - c * per;
-            const int k = r / npair, q = r - k * npair;
+            const int c = dper(e), r = e - c * per;
+            const int k = dnp(r), q = r - k * npair;
             const bool hasb = 2 * q + 1 < nyt;
             cpx<T> xa[kZDirect], xb[kZDirect], oa[kZDirect], ob[kZDirect];
             #pragma unroll

@@ -56,10 +56,11 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     constexpr int QC = (RZ * N + kThreads - 1) / kThreads;
     constexpr int PFC = 0;
     constexpr int CH = QC < gather_chunk<T>() ? QC : gather_chunk<T>();
+    const Div dhw(hw);
     auto cell_at = [&](int e, int &zl, int &yy, int &j) {
         zl = ZF ? e / plane : 0;   // one layer per tile unless z is folded in
         const int r = e - zl * plane;
-        yy = r / hw;
+        yy = dhw(r);
         j = r - yy * hw;
     };
     V3<T> cmi[CH];
@@ -87,14 +88,15 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
         }
     }
     const int per = H * npair;
+    const Div dper(per), dnp(npair);
     if (!zf) {
         const long cstride = (long)H * nz * ny;
         const cpx<T> *src = specb + in.a_off;
         staged_copy<3 * H * RZ, kThreads, Pair2<cpx<T>>>(
             3 * per,
             [&](int e) {
-                const int c = e / per, r = e - c * per;
-                const int k = r / npair, q = r - k * npair;
+                const int c = dper(e), r = e - c * per;
+                const int k = dnp(r), q = r - k * npair;
                 const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
                 if (p16) return load_pair16(p);
                 Pair2<cpx<T>> v;
@@ -106,8 +108,8 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
                 return v;
             },
             [&](int e, Pair2<cpx<T>> v) {
-                const int c = e / per, r = e - c * per;
-                const int k = r / npair, q = r - k * npair;
+                const int c = dper(e), r = e - c * per;
+                const int k = dnp(r), q = r - k * npair;
                 const cpx<T> a = v.a;
                 const T bx = v.b.x, by = v.b.y;
                 cpx<T> *row = s + (c * RZ + q) * N;

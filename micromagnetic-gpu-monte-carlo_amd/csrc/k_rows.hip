@@ -72,9 +72,10 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     constexpr int CH1 = PF ? QC1 : (QC1 < gather_chunk<T>() ? QC1 : gather_chunk<T>());
     const int ncell0 = 4 * NZ * hw, ncell1 = 2 * NZ * hw;
     // cell e = ((r - r0)*NZ + z)*hw + j; y < 0: no cell
+    const Div dhw(hw);
     auto cell0 = [&](int q, int &r, int &z, int &y, int &x) {
         const int e = threadIdx.x + q * NT;
-        const int rz = e / hw, j = e - rz * hw;
+        const int rz = dhw(e), j = e - rz * hw;
         r = rz / NZ - 1;
         z = rz - (r + 1) * NZ;
         y = q < QC0 && e < ncell0 ? row_of(r) : -1;
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sweep(const Inst *__restric
     };
     auto cell1 = [&](int q, int &r, int &z, int &y, int &x) {
         const int e = threadIdx.x + q * NT;
-        const int rz = e / hw, j = e - rz * hw;
+        const int rz = dhw(e), j = e - rz * hw;
         r = rz / NZ;
         z = rz - r * NZ;
         y = q < QC1 && e < ncell1 ? row_of(r) : -1;

@@ -229,9 +229,9 @@ template <typename T>
 __device__ __forceinline__ void kmul_conj(cpx<T> *px, cpx<T> *py, cpx<T> *pz, const cpx<T> *k6)
 {
     const cpx<T> mx = *px, my = *py, mz = *pz;
-    const cpx<T> hx = cadd(cadd(cmul(mx, k6[0]), cmul(my, k6[1])), cmul(mz, k6[2]));
-    const cpx<T> hy = cadd(cadd(cmul(mx, k6[1]), cmul(my, k6[3])), cmul(mz, k6[4]));
-    const cpx<T> hz = cadd(cadd(cmul(mx, k6[2]), cmul(my, k6[4])), cmul(mz, k6[5]));
+    const cpx<T> hx = cdot3(mx, k6[0], my, k6[1], mz, k6[2]);
+    const cpx<T> hy = cdot3(mx, k6[1], my, k6[3], mz, k6[4]);
+    const cpx<T> hz = cdot3(mx, k6[2], my, k6[4], mz, k6[5]);
     *px = cconj(hx);
     *py = cconj(hy);
     *pz = cconj(hz);
@@ -321,8 +321,9 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     }
     __syncthreads();
     fft_fixed<T, N, ROWS>(s, tw);
+    const Div dLy(Ly);
     for (int e = threadIdx.x; e < ROWS * Ly; e += kThreads) {
-        const int r = e / Ly, y = e - r * Ly;
+        const int r = dLy(e), y = e - r * Ly;
         const int comp = r / C, col = r % C;
         if (col < ncol)
             buf[comp * cstride + ((long)(kx0 + col) * Pz + kz) * Ly + y] = cconj(s[r * N + y]);
@@ -473,17 +474,14 @@ void k_yzconv_n(const Inst *__restrict__ I,
             for (int zp = 0; zp < NZ; ++zp) {
                 const cpx<T> *k6 = kq[z - zp + NZ - 1];
                 const cpx<T> mx = mv[zp][0], my = mv[zp][1], mz = mv[zp][2];
-                const cpx<T> hx = cadd(cadd(cmul(mx, k6[0]), cmul(my, k6[1])), cmul(mz, k6[2]));
-                const cpx<T> hy = cadd(cadd(cmul(mx, k6[1]), cmul(my, k6[3])), cmul(mz, k6[4]));
-                const cpx<T> hz = cadd(cadd(cmul(mx, k6[2]), cmul(my, k6[4])), cmul(mz, k6[5]));
                 if (zp == 0) {
-                    h[0] = hx;
-                    h[1] = hy;
-                    h[2] = hz;
+                    h[0] = cdot3(mx, k6[0], my, k6[1], mz, k6[2]);
+                    h[1] = cdot3(mx, k6[1], my, k6[3], mz, k6[4]);
+                    h[2] = cdot3(mx, k6[2], my, k6[4], mz, k6[5]);
                 } else {
-                    h[0] = cadd(h[0], hx);
-                    h[1] = cadd(h[1], hy);
-                    h[2] = cadd(h[2], hz);
+                    h[0] = cdot3_acc(h[0], mx, k6[0], my, k6[1], mz, k6[2]);
+                    h[1] = cdot3_acc(h[1], mx, k6[1], my, k6[3], mz, k6[4]);
+                    h[2] = cdot3_acc(h[2], mx, k6[2], my, k6[4], mz, k6[5]);
                 }
             }
             #pragma unroll
@@ -492,8 +490,9 @@ void k_yzconv_n(const Inst *__restrict__ I,
     }
     __syncthreads();
     fft_fixed<T, N, ROWS, NT, true>(s, tw);
+    const Div dLy(Ly);
     for (int e = threadIdx.x; e < ROWS * Ly; e += NT) {
-        const int r = e / Ly, y = e - r * Ly;
+        const int r = dLy(e), y = e - r * Ly;
         const int z = r / 3, c = r - 3 * z;
         A[c * astride + ((long)kx * NZ + z) * Ly + y] = cconj(s[r * N + y]);
     }
@@ -548,9 +547,9 @@ __global__ __launch_bounds__(kThreads) void k_ymul(const Inst *__restrict__ I,
         cpx<T> *py = s + (1 * ncol + col) * Py + ky;
         cpx<T> *pz = s + (2 * ncol + col) * Py + ky;
         const cpx<T> mx = *px, my = *py, mz = *pz;
-        const cpx<T> hx = cadd(cadd(cmul(mx, kxx), cmul(my, kxy)), cmul(mz, kxz));
-        const cpx<T> hy = cadd(cadd(cmul(mx, kxy), cmul(my, kyy)), cmul(mz, kyz));
-        const cpx<T> hz = cadd(cadd(cmul(mx, kxz), cmul(my, kyz)), cmul(mz, kzz));
+        const cpx<T> hx = cdot3(mx, kxx, my, kxy, mz, kxz);
+        const cpx<T> hy = cdot3(mx, kxy, my, kyy, mz, kyz);
+        const cpx<T> hz = cdot3(mx, kxz, my, kyz, mz, kzz);
         *px = cconj(hx);
         *py = cconj(hy);
         *pz = cconj(hz);
