@@ -90,15 +90,33 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     const int per = H * npair;
     const Div dper(per), dnp(npair);
     if (!zf) {
+        // the spectrum as the y pass left it: y-contiguous in A, or for films
+        // whose y pass sums z directly (Inst::a2) pair-major in the B region,
+        // [3][nz][ny/2][Hx][2], where a row pair's spectrum is contiguous
+        // (whole lines instead of 16 bytes of each)
+        const bool pm = in.a2;
         const long cstride = (long)H * nz * ny;
-        const cpx<T> *src = specb + in.a_off;
+        const cpx<T> *src = specb + (pm ? in.b_off : in.a_off);
+        const int np2 = ny >> 1;
+        const bool p16x = pm ? sizeof(T) == 4 : p16;
+        // element e -> (component c, frequency k, pair q): pairs innermost
+        // for the y-contiguous layout, frequencies innermost for pair-major
+        auto at = [&](int e, int &c, int &k, int &q) {
+            c = dper(e);
+            const int r = e - c * per;
+            const int qa = r / H, qb = dnp(r);
+            q = pm ? qa : r - qb * npair;
+            k = pm ? r - qa * H : qb;
+        };
         staged_copy<3 * H * RZ, kThreads, Pair2<cpx<T>>>(
             3 * per,
             [&](int e) {
-                const int c = dper(e), r = e - c * per;
-                const int k = dnp(r), q = r - k * npair;
-                const cpx<T> *p = src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
-                if (p16) return load_pair16(p);
+                int c, k, q;
+                at(e, c, k, q);
+                const cpx<T> *p =
+                    pm ? src + ((((long)c * nz + z0) * np2 + (y0 >> 1) + q) * H + k) * 2
+                       : src + c * cstride + ((long)k * nz + z0) * ny + y0 + 2 * q;
+                if (p16x) return load_pair16(p);
                 Pair2<cpx<T>> v;
                 // branch-free: p[1] read in place of a missing row, zero selected
                 const bool hb = 2 * q + 1 < nyt;
@@ -108,8 +126,8 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
                 return v;
             },
             [&](int e, Pair2<cpx<T>> v) {
-                const int c = dper(e), r = e - c * per;
-                const int k = dnp(r), q = r - k * npair;
+                int c, k, q;
+                at(e, c, k, q);
                 const cpx<T> a = v.a;
                 const T bx = v.b.x, by = v.b.y;
                 cpx<T> *row = s + (c * RZ + q) * N;

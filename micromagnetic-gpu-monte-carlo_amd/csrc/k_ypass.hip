@@ -385,7 +385,7 @@ void k_yzconv_n(const Inst *__restrict__ I,
                                                        const Work *__restrict__ W,
                                                        cpx<T> *__restrict__ Ab,
                                                        const cpx<T> *__restrict__ khatb,
-                                                       const cpx<T> *__restrict__ twb)
+                                                       const cpx<T> *__restrict__ twb, int a2)
 {
     constexpr int NT = kYzThreads;
     constexpr int ROWS = 3 * NZ;
@@ -490,6 +490,28 @@ void k_yzconv_n(const Inst *__restrict__ I,
     }
     __syncthreads();
     fft_fixed<T, N, ROWS, NT, true>(s, tw);
+    if (a2 && in.a2) {
+        // pair-major, for the inverse x pass of a fused sweep: element
+        // (c, z, y) at B[(((c*NZ + z)*ny/2 + y/2)*Hx + kx)*2 + (y & 1)], so a
+        // row pair's spectrum is contiguous; the items of 8 consecutive x
+        // frequencies run on one XCD and fill each 128-byte line together
+        cpx<T> *A2 = Ab + in.b_off;
+        const int np = Ly >> 1;
+        const Div dnp(np);
+        for (int e = threadIdx.x; e < ROWS * np; e += NT) {
+            const int r = dnp(e), p = e - r * np;
+            const int z = r / 3, c = r - 3 * z;
+            const cpx<T> a = cconj(s[r * N + 2 * p]), b = cconj(s[r * N + 2 * p + 1]);
+            cpx<T> *o = A2 + ((((long)c * NZ + z) * np + p) * H + kx) * 2;
+            if (sizeof(T) == 4) {
+                store_pair16(o, a, b);
+            } else {
+                o[0] = a;
+                o[1] = b;
+            }
+        }
+        return;
+    }
     const Div dLy(Ly);
     for (int e = threadIdx.x; e < ROWS * Ly; e += NT) {
         const int r = dLy(e), y = e - r * Ly;
@@ -652,13 +674,14 @@ hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int 
 
 template <typename T>
 hipError_t launch_yzconv(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n, int nz,
-                         cpx<T> *A, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+                         cpx<T> *A, const cpx<T> *khat, const cpx<T> *tw, size_t lds, bool a2)
 {
     if (n <= 0) return hipSuccess;
     const dim3 g(n);
 #define MMC_YZC(NN, ZZ)                                                                        \
     if (fixed_n == NN && nz == ZZ) {                                                           \
-        MMC_LAUNCH((k_yzconv_n<T, NN, ZZ>), g, dim3(kYzThreads), lds, st, I, W, A, khat, tw);  \
+        MMC_LAUNCH((k_yzconv_n<T, NN, ZZ>), g, dim3(kYzThreads), lds, st, I, W, A, khat, tw,   \
+                   (int)a2);                                                                   \
         return post();                                                                         \
     }
 #define MMC_YZC_N(NN) MMC_YZC(NN, 2) MMC_YZC(NN, 3) MMC_YZC(NN, 4)
@@ -717,7 +740,8 @@ hipError_t set_lds_ypass(int bytes)
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                        cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_yzconv<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
-                                         cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);    \
+                                         cpx<T> *, const cpx<T> *, const cpx<T> *, size_t,     \
+                                         bool);                                                \
     template hipError_t launch_yfwd_tensor<T>(hipStream_t, const Inst *, const Work *, int,   \
                                               const cpx<double> *, cpx<T> *,                  \
                                               const cpx<double> *, size_t);                   \

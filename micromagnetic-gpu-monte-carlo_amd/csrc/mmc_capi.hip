@@ -695,6 +695,9 @@ void build_lists(mmc_ctx *c)
         c->xl.seg.push_back(sg);
     }
     for (auto &kv : zditems) {
+        // 8 consecutive x frequencies on one XCD: their pair-major stores
+        // (16 bytes each) fill 128-byte lines together
+        xcd_group(kv.second, 8);
         Segment sg = zdclass[kv.first];
         sg.begin = c->yzl.n();
         sg.count = (int)kv.second.size();
@@ -860,8 +863,13 @@ int alloc_all(mmc_ctx *c)
             in.twz = (long)tw; twiddles(in.Pz, tw32); twiddles(in.Pz, tw64); tw += in.Pz;
         }
     }
-    for (auto &in : c->inst)
+    for (auto &in : c->inst) {
         if (in.cal_demag) in.b_off = in.b_off < 0 ? in.a_off : (long)a + in.b_off;
+        // the B region of a zd instance (no z transforms) is free; pair-major
+        // layout of [3][nz][ny/2][Hx][2] fits it when ny is even (Pz > nz)
+        in.a2 = in.cal_demag && in.zd && (in.ny % 2) == 0 && in.b_off != in.a_off &&
+                (in.b_off % 2) == 0;
+    }
     c->m_elems = m; c->a_elems = a; c->b_elems = b; c->khat_elems = kh; c->k_elems = k;
     c->ka_elems = ka;
     HIPCHK(c, hipMalloc(&c->d_m, std::max<size_t>(1, m) * ts));
@@ -1108,7 +1116,7 @@ int pass_zfwd(mmc_ctx *c, bool sweep)
 // sweeps (`sweep`) instances whose fused tiles fold z in skip the z passes:
 // their tiles read and write B directly.
 template <typename T>
-int pass_zyz(mmc_ctx *c, bool sweep = false)
+int pass_zyz(mmc_ctx *c, bool sweep = false, bool a2 = false)
 {
     const Bufs<T> b(c);
     hipStream_t st = c->stream;
@@ -1122,7 +1130,7 @@ int pass_zyz(mmc_ctx *c, bool sweep = false)
     if (rc) return rc;
     rc = run_segments(c, c->yzl, K_YZC, [&](const Segment &sg, hipStream_t s) {
         return launch_yzconv<T>(s, c->dInst, c->yzl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows,
-                                b.A, b.Kh, b.tw, sg.lds);
+                                b.A, b.Kh, b.tw, sg.lds, a2);
     });
     if (rc) return rc;
     if (l.n())
@@ -1172,7 +1180,9 @@ int enqueue_sweep(mmc_ctx *c, uint32_t sweep_off)
         });
     }
     if (fused_sweeps(c)) {
-        int rc = pass_zyz<T>(c, true);
+        // films whose y pass sums z directly hand the inverse x pass their
+        // spectrum pair-major (Inst::a2)
+        int rc = pass_zyz<T>(c, true, true);
         if (rc) return rc;
         const Bufs<T> b(c);
         bool zf = false;

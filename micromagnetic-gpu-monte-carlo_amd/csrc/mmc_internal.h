@@ -54,6 +54,10 @@ struct Inst {
     int zf;               // fused tiles span all layers and transform z too
     int zd;               // z convolution summed directly in the y pass
                           // (k_yzconv_n); Khat holds x-y spectra of z slices
+    int a2;               // zd, even ny: in fused sweeps the y pass writes the x
+                          // spectrum pair-major into the B region,
+                          // [3][nz][ny/2][Hx][2] (a row pair's spectrum
+                          // contiguous), which the inverse x pass reads
     int kmode[6];         // Khat storage of each component: 0 zero, 1 real
                           // (imaginary part below rounding), 2 complex
     long N, NP;
@@ -132,10 +136,12 @@ constexpr int kZConvMax = 4;
 // slices (Inst::zd): y transform, Khat product with the z convolution
 // summed directly, y inverse, in one pass over A; items (inst, kx), one
 // launch per (fixed_n, nz).
+// a2: write the result pair-major into the B region of instances with
+// Inst::a2 (fused sweeps, whose inverse x pass reads it there).
 template <typename T>
 hipError_t launch_yzconv(hipStream_t st, const Inst *I, const Work *W, int n,
                          int fixed_n, int nz, cpx<T> *A, const cpx<T> *khat,
-                         const cpx<T> *tw, size_t lds);
+                         const cpx<T> *tw, size_t lds, bool a2);
 template <typename TO>
 hipError_t launch_yfwd_tensor(hipStream_t st, const Inst *I, const Work *W,
                               int n, const cpx<double> *ka, cpx<TO> *khat,
