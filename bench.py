@@ -287,6 +287,34 @@ def cpu_baseline(mod, ctx, kws, sweeps):
                        f"{dt:.1f} s) on the C oracle (OpenMP), GPU-built tensors")
 
 
+def tensor_cpu_baseline(mod):
+    """The demag tensor with periodic-image sums built by the FP64 C oracle
+    (OpenMP, all host cores) and by the GPU, same geometry: the 256x256 film
+    with 2 periodic images per axis (5x5 image sums; the film's own 20 images
+    take the oracle minutes), the one-off cost the tensor cache saves."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    kw = film256(pbc_x=2, pbc_y=2)
+    o = oracle.Oracle(**kw)
+    t0 = time.perf_counter()
+    o.build_tensor()
+    t_cpu = time.perf_counter() - t0
+    c = mod.Context(mod.make_params(**kw), mod.FP64)
+    c.build_tensor()          # first build: loads and warms the kernels
+    c.sync()
+    t0 = time.perf_counter()
+    c.build_tensor()
+    c.sync()
+    t_gpu = time.perf_counter() - t0
+    err = float(np.abs(c.tensor() - o.tensor()).max() / np.abs(o.tensor()).max())
+    c.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(sample="FP64 demag tensor of the 256x256x1 film with 2 periodic images per axis",
+                cpu_s=t_cpu, cpu_cores=cores, kind="port", gpu_s=t_gpu, gpu_over_cpu=t_cpu / t_gpu,
+                max_rel_diff=err)
+
+
 def anneal_stages(T0=1400.0, T1=40.0, f=0.9):
     out, T = [], T0
     while T > T1:
@@ -436,6 +464,9 @@ def main():
     if a.cpu_sweeps > 0:
         log("[bench] CPU baseline (oracle)")
         cpu = cpu_baseline(mod, r32["ctx"], kws, a.cpu_sweeps)
+        if not a.no_extra:
+            log("[bench] tensor build: oracle vs GPU")
+            extra["tensor_build_cpu_baseline"] = tensor_cpu_baseline(mod)
     line = {
         "metric": METRIC,
         "value": r32["attempts_per_s"],
