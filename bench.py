@@ -307,12 +307,14 @@ def tensor_cpu_baseline(mod):
     c.build_tensor()
     c.sync()
     t_gpu = time.perf_counter() - t0
-    err = float(np.abs(c.tensor() - o.tensor()).max() / np.abs(o.tensor()).max())
+    # norm-wise: single far entries carry the Newell cancellation noise of two
+    # FP64 libm implementations, which grows with the image distance
+    err = float(np.linalg.norm(c.tensor() - o.tensor()) / np.linalg.norm(o.tensor()))
     c.close()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(sample="FP64 demag tensor of the 256x256x1 film with 2 periodic images per axis",
                 cpu_s=t_cpu, cpu_cores=cores, kind="port", gpu_s=t_gpu, gpu_over_cpu=t_cpu / t_gpu,
-                max_rel_diff=err)
+                normwise_rel_diff=err)
 
 
 def anneal_stages(T0=1400.0, T1=40.0, f=0.9):
