@@ -249,6 +249,10 @@ STRIPS = [
     ("y768", dict(nx=5, ny=384, nz=1)),                  # Py = 768
     ("y2048", dict(nx=4, ny=1024, nz=1)),                # Py = 2048
     ("yz1024_2layers", dict(nx=8, ny=512, nz=2)),        # direct z, Py = 1024
+    # direct z with fused x-pass lengths and even ny: in fused sweeps the y
+    # pass hands the inverse x pass its spectrum pair-major
+    ("yz128_2layers_fused", dict(nx=256, ny=64, nz=2)),  # Px = 512, Py = 128, Pz = 3
+    ("yz64_3layers_fused", dict(nx=128, ny=64, nz=3, pbc_y=1)),  # Px = 256, Py = 64 periodic, Pz = 5
 ]
 
 
