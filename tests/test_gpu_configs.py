@@ -244,6 +244,10 @@ STRIPS = [
     ("x1024", dict(nx=512, ny=4, nz=1)),                 # Px = 1024
     ("x2048", dict(nx=1024, ny=2, nz=1)),                # Px = 2048
     ("x1024_pbc_2layers", dict(nx=1024, ny=4, nz=2, pbc_x=1)),
+    # an odd row count (the last tile holds one row), and two periodic rows
+    # (both y neighbours of a cell are the pair's other row)
+    ("x1024_odd_rows", dict(nx=512, ny=5, nz=1)),
+    ("x1024_pbc_two_rows", dict(nx=512, ny=2, nz=1, pbc_y=1)),
     ("y384", dict(nx=6, ny=192, nz=1)),                  # Py = 384
     ("y512", dict(nx=6, ny=256, nz=1)),                  # Py = 512
     ("y768", dict(nx=5, ny=384, nz=1)),                  # Py = 768
