@@ -65,18 +65,40 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     };
     V3<T> cmi[CH];
     Nbrs<T> cnb[CH];
-    Move<T> cmv[CH];
-    auto fetch = [&](int q, int k) {
-        const int e = threadIdx.x + q * kThreads;
+    // Trial moves depend on the cell and the counters only.  Tiles of more
+    // than two cells per thread whose spectrum comes from A draw all of them
+    // while the spectrum loads are in flight (EM), when the vector ALU would
+    // otherwise wait, and keep them across the transform; the others draw
+    // each with the cell's other inputs.
+    constexpr bool EM = QC > 2;
+    const bool em = EM && !zf;
+    Move<T> cmv[EM ? QC : CH];
+    auto cell_id = [&](int e, int &xa, int &y, int &z) {
         int zl, yy, j;
         cell_at(e, zl, yy, j);
-        const int z = z0 + zl, y = y0 + yy;
-        const int xa = 2 * j + ((y + z) & 1);
+        z = z0 + zl;
+        y = y0 + yy;
+        xa = 2 * j + ((y + z) & 1);
+    };
+    auto draw_all = [&] {
+        #pragma unroll
+        for (int q = 0; q < (EM ? QC : 0); ++q) {
+            const int e = threadIdx.x + q * kThreads;
+            int xa, y, z;
+            cell_id(e, xa, y, z);
+            if (e < ncell && xa < nx)
+                cmv[q] = trial_move<T>(in, (uint32_t)((long)nx * (y + (long)ny * z) + xa), sweep, stage);
+        }
+    };
+    auto fetch = [&](int q, int k) {
+        const int e = threadIdx.x + q * kThreads;
+        int xa, y, z;
+        cell_id(e, xa, y, z);
         if (e < ncell && xa < nx) {
             const long i = (long)nx * (y + (long)ny * z) + xa;
             cmi[k] = load_m(m, NN, i);
             cnb[k] = gather_nbrs(in, m, xa, y, z);
-            cmv[k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
+            if (!em) cmv[EM ? q : k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
         }
     };
     if (npair < R) {
@@ -133,7 +155,8 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
                 cpx<T> *row = s + (c * RZ + q) * N;
                 row[k] = {a.x - by, -(a.y + bx)};
                 if (k > 0 && N - k >= H) row[N - k] = {a.x + by, a.y - bx};
-            });
+            },
+            draw_all);
     } else {
         const int Pz = in.Pz;
         const long cstride = (long)H * Pz * ny;
@@ -200,7 +223,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
                 hc[2 * cst + o] = h1.z;
             }
             if (xa < nx)
-                metro_apply(in, m, rowi + xa, cmi[k], cnb[k], hval(zl, yy, xa), cmv[k], bt);
+                metro_apply(in, m, rowi + xa, cmi[k], cnb[k], hval(zl, yy, xa), cmv[EM ? q0 + k : k], bt);
         }
     }
 }

@@ -75,8 +75,20 @@ extern __shared__ __align__(16) unsigned char g_smem[];
 // pays one global latency per element; here every load of a chunk is issued
 // before the chunk's first write, so the latencies overlap.  Chunks hold at
 // most 128 bytes of loaded values per thread.
+template <int MAXC, int NT, typename V, typename LD, typename ST, typename MID>
+__device__ __forceinline__ void staged_copy(int count, LD ld, ST st, MID mid);
+
 template <int MAXC, int NT, typename V, typename LD, typename ST>
 __device__ __forceinline__ void staged_copy(int count, LD ld, ST st)
+{
+    staged_copy<MAXC, NT, V>(count, ld, st, [] {});
+}
+
+// The same, with mid() run once after the first chunk's loads are issued
+// and before their values are used: work that needs none of them (drawing
+// random numbers, say) fills the loads' latency.
+template <int MAXC, int NT, typename V, typename LD, typename ST, typename MID>
+__device__ __forceinline__ void staged_copy(int count, LD ld, ST st, MID mid)
 {
     constexpr int Q = (MAXC + NT - 1) / NT;
     constexpr int CHM = 128 / (int)sizeof(V) > 0 ? 128 / (int)sizeof(V) : 1;
@@ -89,6 +101,7 @@ __device__ __forceinline__ void staged_copy(int count, LD ld, ST st)
             const int e = threadIdx.x + (q0 + q) * NT;
             if (q0 + q < Q && e < count) v[q] = ld(e);
         }
+        if (q0 == 0) mid();
         #pragma unroll
         for (int q = 0; q < CH; ++q) {
             const int e = threadIdx.x + (q0 + q) * NT;
