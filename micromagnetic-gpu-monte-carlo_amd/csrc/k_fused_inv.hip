@@ -65,14 +65,13 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     };
     V3<T> cmi[CH];
     Nbrs<T> cnb[CH];
-    // Trial moves depend on the cell and the counters only.  Tiles of more
-    // than two cells per thread whose spectrum comes from A draw all of them
-    // while the spectrum loads are in flight (EM), when the vector ALU would
-    // otherwise wait, and keep them across the transform; the others draw
+    // Trial moves depend on the cell and the counters only: tiles whose
+    // spectrum comes from A (or B, pair-major) draw all of theirs while the
+    // spectrum loads are in flight, when the vector ALU would otherwise
+    // wait, and keep them across the transform; tiles that fold z in draw
     // each with the cell's other inputs.
-    constexpr bool EM = QC > 2;
-    const bool em = EM && !zf;
-    Move<T> cmv[EM ? QC : CH];
+    const bool em = !zf;
+    Move<T> cmv[QC];
     auto cell_id = [&](int e, int &xa, int &y, int &z) {
         int zl, yy, j;
         cell_at(e, zl, yy, j);
@@ -82,7 +81,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     };
     auto draw_all = [&] {
         #pragma unroll
-        for (int q = 0; q < (EM ? QC : 0); ++q) {
+        for (int q = 0; q < QC; ++q) {
             const int e = threadIdx.x + q * kThreads;
             int xa, y, z;
             cell_id(e, xa, y, z);
@@ -98,7 +97,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
             const long i = (long)nx * (y + (long)ny * z) + xa;
             cmi[k] = load_m(m, NN, i);
             cnb[k] = gather_nbrs(in, m, xa, y, z);
-            if (!em) cmv[EM ? q : k] = trial_move<T>(in, (uint32_t)i, sweep, stage);
+            if (!em) cmv[q] = trial_move<T>(in, (uint32_t)i, sweep, stage);
         }
     };
     if (npair < R) {
@@ -223,7 +222,7 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
                 hc[2 * cst + o] = h1.z;
             }
             if (xa < nx)
-                metro_apply(in, m, rowi + xa, cmi[k], cnb[k], hval(zl, yy, xa), cmv[EM ? q0 + k : k], bt);
+                metro_apply(in, m, rowi + xa, cmi[k], cnb[k], hval(zl, yy, xa), cmv[q0 + k], bt);
         }
     }
 }
