@@ -88,7 +88,45 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
         return cpx<T>{re, im};
     };
     auto row_st = [&](int e, cpx<T> v) { s[e] = v; };
-    if (PFC) {
+    // Tiles of one complete row pair of one layer with more than two cells
+    // per thread (LNB) issue, with the row loads, every colour-1 input that
+    // is not in the tile's rows: the compact h, the neighbour row outside
+    // the pair, the z neighbours (as their sum, the only form the bond sums
+    // use); the x neighbours and the pair's other row are then read from the
+    // rows in LDS (colour-0 cells, which this pass does not write), so the
+    // Metropolis phase issues no global load.  Missing neighbours are zero,
+    // as in gather_nbrs.
+    constexpr bool LNB = PFC == 0 && RZ == 1 && QC > 2 && QC <= 4;
+    const bool lnb = LNB && !zf && nyt == 2;
+    V3<T> ph[LNB ? QC : 1], po[LNB ? QC : 1], pz[LNB ? QC : 1];
+    auto keep = [](bool ok, const V3<T> &v) -> V3<T> {
+        return {ok ? v.x : T(0), ok ? v.y : T(0), ok ? v.z : T(0)};
+    };
+    const uint32_t unx = (uint32_t)nx, uny = (uint32_t)ny;
+    if (LNB && lnb) {
+        const auto rows = staged_load<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld);
+        const int jzp = nz > 1 ? nbr(z0, +1, nz, in.pbc[2]) : -1;
+        const int jzm = nz > 1 ? nbr(z0, -1, nz, in.pbc[2]) : -1;
+        #pragma unroll
+        for (int q = 0; q < (LNB ? QC : 0); ++q) {
+            const int e = threadIdx.x + q * kThreads;
+            int zl, yy, j;
+            cell_at(e, zl, yy, j);
+            const int y = y0 + yy;
+            const int x = 2 * j + ((y + z0 + 1) & 1);
+            // a slot without a cell reads cell (0, y) and its values go unused
+            const int xs = (e < ncell && x < nx) ? x : 0;
+            const long oh = ((long)z0 * ny + y) * hw + j;
+            ph[q] = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
+            const int jo = nbr(y, yy ? +1 : -1, ny, in.pbc[1]);
+            po[q] = keep(jo >= 0, load_m(m, (uint32_t)NN, cell_index(unx, uny, xs, jo >= 0 ? jo : y, z0)));
+            V3<T> zp = {T(0), T(0), T(0)}, zm = zp;
+            if (jzp >= 0) zp = load_m(m, (uint32_t)NN, cell_index(unx, uny, xs, y, jzp));
+            if (jzm >= 0) zm = load_m(m, (uint32_t)NN, cell_index(unx, uny, xs, y, jzm));
+            pz[q] = {zp.x + zm.x, zp.y + zm.y, zp.z + zm.z};
+        }
+        staged_store<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, rows, row_st);
+    } else if (PFC) {
         const auto rows = staged_load<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld);
         #pragma unroll
         for (int q = 0; q < PFC; ++q) fetch(q, q);
@@ -98,7 +136,45 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     }
     store_twiddles<T, N>(tw, twr);
     __syncthreads();
-    {
+    if (LNB && lnb) {
+        const T bt = T(beta[w.inst]);
+        auto lds_m = [&](int yy, int x) -> V3<T> {
+            const cpx<T> a = s[x], b = s[N + x], c = s[2 * N + x];
+            return yy ? V3<T>{a.y, b.y, c.y} : V3<T>{a.x, b.x, c.x};
+        };
+        #pragma unroll
+        for (int q = 0; q < (LNB ? QC : 0); ++q) {
+            const int e = threadIdx.x + q * kThreads;
+            if (e >= ncell) break;
+            int zl, yy, j;
+            cell_at(e, zl, yy, j);
+            const int y = y0 + yy;
+            const int x = 2 * j + ((y + z0 + 1) & 1);
+            if (x >= nx) continue;
+            Nbrs<T> nb;
+            const int jp = nbr(x, +1, nx, in.pbc[0]), jm = nbr(x, -1, nx, in.pbc[0]);
+            nb.xp = keep(jp >= 0, lds_m(yy, jp >= 0 ? jp : x));
+            nb.xm = keep(jm >= 0, lds_m(yy, jm >= 0 ? jm : x));
+            const V3<T> vin = lds_m(yy ^ 1, x);
+            nb.yp = yy ? po[q] : vin;
+            nb.ym = yy ? vin : po[q];
+            nb.zp = pz[q];
+            nb.zm = {T(0), T(0), T(0)};
+            const V3<T> mi = lds_m(yy, x);
+            const long i = (long)nx * (y + (long)ny * z0) + x;
+            const Move<T> mv = trial_move<T>(in, (uint32_t)i, sweep, stage);
+            const V3<T> mn = metro_apply(in, m, i, mi, nb, ph[q], mv, bt);
+            if (yy) {
+                s[x].y = mn.x;
+                s[N + x].y = mn.y;
+                s[2 * N + x].y = mn.z;
+            } else {
+                s[x].x = mn.x;
+                s[N + x].x = mn.y;
+                s[2 * N + x].x = mn.z;
+            }
+        }
+    } else {
         const T bt = T(beta[w.inst]);
         #pragma unroll
         for (int q0 = 0; q0 < QC; q0 += CH) {
