@@ -241,11 +241,17 @@ __device__ __forceinline__ void kmul_conj(cpx<T> *px, cpx<T> *py, cpx<T> *pz, co
 // buffer of 3*C rows (row comp*C + col), the item's Khat values prefetched
 // into registers before the forward transform so their HBM latency overlaps
 // it.
-template <typename T, int N, int C>
+template <typename T, int N, int C, bool HALF>
 __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T> *__restrict__ bufb,
                                            const cpx<T> *__restrict__ khatb,
                                            const cpx<T> *__restrict__ twb)
 {
+    // HALF: the instance's ny rows fit in the lower half of the padded
+    // length (an open y axis, N = 2ny), so the forward transform reads only
+    // that half (its upper half is the zero padding) and the inverse forms
+    // only that half
+    constexpr bool OLO = HALF && fixed_olo_ok(N);
+    constexpr int NL = HALF ? N / 2 : N;             // rows' elements loaded per column
     constexpr int ROWS = 3 * C;
     constexpr int Q = (C * N + kThreads - 1) / kThreads;
     // the k points of a thread are prefetched across the forward transform
@@ -275,8 +281,8 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     // issue order): twiddles and spectrum rows, then the tensor values,
     // which may still be in flight during the forward transform
     const auto twr = load_twiddles<T, N>(twb + in.twy);
-    const auto rows = staged_load<ROWS * N, kThreads, cpx<T>>(ROWS * N, [&](int e) {
-        const int r = e / N, y = e % N;
+    const auto rows = staged_load<ROWS * NL, kThreads, cpx<T>>(ROWS * NL, [&](int e) {
+        const int r = e / NL, y = e % NL;
         const int comp = r / C, col = r % C;
         // component-wise select: a conditionally loaded struct is kept in scratch
         T vx = T(0), vy = T(0);
@@ -292,10 +298,12 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
         #pragma unroll
         for (int q = 0; q < (PF ? Q : 1); ++q) load_k6(q, k6p[q]);
     }
-    staged_store<ROWS * N, kThreads, cpx<T>>(ROWS * N, rows, [&](int e, cpx<T> v) { s[e] = v; });
+    staged_store<ROWS * NL, kThreads, cpx<T>>(ROWS * NL, rows, [&](int e, cpx<T> v) {
+        s[(e / NL) * N + e % NL] = v;
+    });
     store_twiddles<T, N>(tw, twr);
     __syncthreads();
-    fft_fixed<T, N, ROWS>(s, tw);
+    fft_fixed<T, N, ROWS, kThreads, false, HALF>(s, tw);
     // without prefetch, all k points' values are loaded before the first product
     cpx<T> k6l[PF ? 1 : Q][6];
     if (!PF) {
@@ -320,7 +328,7 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
         }
     }
     __syncthreads();
-    fft_fixed<T, N, ROWS>(s, tw);
+    fft_fixed<T, N, ROWS, kThreads, false, false, OLO>(s, tw);
     const Div dLy(Ly);
     for (int e = threadIdx.x; e < ROWS * Ly; e += kThreads) {
         const int r = dLy(e), y = e - r * Ly;
@@ -334,7 +342,7 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
 // columns (one launch per distinct (length, C); a kernel of its own keeps its
 // register count low).  C is fixed_cols(N), or 1 for grids too small to fill
 // the chip with wide items.
-template <typename T, int N, int C>
+template <typename T, int N, int C, bool HALF>
 __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
                                                      const Work *__restrict__ W,
                                                      cpx<T> *__restrict__ bufb,
@@ -342,12 +350,12 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
                                                      const cpx<T> *__restrict__ twb)
 {
     const Work w = W[blockIdx.x];
-    ymul_fixed<T, N, C>(I[w.inst], w, bufb, khatb, twb);
+    ymul_fixed<T, N, C, HALF>(I[w.inst], w, bufb, khatb, twb);
 }
 
 // One launch over y-pass items of every y length of a batch (any_y_len),
 // columns per item from tile policy POL (any_tile).
-template <typename T, int POL>
+template <typename T, int POL, bool HALF>
 __global__ __launch_bounds__(kThreads) void k_ymul_any(const Inst *__restrict__ I,
                                                        const Work *__restrict__ W,
                                                        cpx<T> *__restrict__ bufb,
@@ -357,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void k_ymul_any(const Inst *__restrict__ 
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
     switch (in.Py) {
-#define MMC_ANY(NN) case NN: ymul_fixed<T, NN, any_tile(POL, NN)>(in, w, bufb, khatb, twb); break;
+#define MMC_ANY(NN) case NN: ymul_fixed<T, NN, any_tile(POL, NN), HALF>(in, w, bufb, khatb, twb); break;
     MMC_ANY(64) MMC_ANY(96) MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
 #undef MMC_ANY
     }
@@ -646,23 +654,30 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, int 
 
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
-                       int cols, cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw, size_t lds)
+                       int cols, cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw, size_t lds,
+                       bool half)
 {
     if (n <= 0) return hipSuccess;
     const dim3 g(n), b(kThreads);
     if (fixed_n == kAnyLen) {
+#define MMC_YA(P)                                                                              \
+    if (half) MMC_LAUNCH((k_ymul_any<T, P, true>), g, b, lds, st, I, W, buf, khat, tw);        \
+    else MMC_LAUNCH((k_ymul_any<T, P, false>), g, b, lds, st, I, W, buf, khat, tw);
         switch (cols) {
-        case 0: MMC_LAUNCH((k_ymul_any<T, 0>), g, b, lds, st, I, W, buf, khat, tw); break;
-        case 1: MMC_LAUNCH((k_ymul_any<T, 1>), g, b, lds, st, I, W, buf, khat, tw); break;
-        case 2: MMC_LAUNCH((k_ymul_any<T, 2>), g, b, lds, st, I, W, buf, khat, tw); break;
-        default: MMC_LAUNCH((k_ymul_any<T, 3>), g, b, lds, st, I, W, buf, khat, tw); break;
+        case 0: MMC_YA(0) break;
+        case 1: MMC_YA(1) break;
+        case 2: MMC_YA(2) break;
+        default: MMC_YA(3) break;
         }
+#undef MMC_YA
         return post();
     }
 #define MMC_YN(NN)                                                                              \
     if (fixed_n == NN) {                                                                        \
-        if (cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1>), g, b, lds, st, I, W, buf, khat, tw);   \
-        else MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN)>), g, b, lds, st, I, W, buf, khat, tw); \
+        if (half && cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1, true>), g, b, lds, st, I, W, buf, khat, tw); \
+        else if (half) MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN), true>), g, b, lds, st, I, W, buf, khat, tw); \
+        else if (cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1, false>), g, b, lds, st, I, W, buf, khat, tw); \
+        else MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN), false>), g, b, lds, st, I, W, buf, khat, tw); \
         return post();                                                                          \
     }
     MMC_YN(64) MMC_YN(96) MMC_YN(128) MMC_YN(192) MMC_YN(256) MMC_YN(384) MMC_YN(512)
@@ -713,16 +728,22 @@ hipError_t set_lds_ypass(int bytes)
     MMC_ZL(16) MMC_ZL(32) MMC_ZL(64)
 #undef MMC_ZL
     set_lds_one(e, (const void *)k_ymul<T>, bytes);
-#define MMC_YL(NN) set_lds_one(e, (const void *)k_ymul_n<T, NN, 1>, bytes); \
-    set_lds_one(e, (const void *)k_ymul_n<T, NN, fixed_cols(NN)>, bytes);
+#define MMC_YL(NN) set_lds_one(e, (const void *)k_ymul_n<T, NN, 1, false>, bytes); \
+    set_lds_one(e, (const void *)k_ymul_n<T, NN, fixed_cols(NN), false>, bytes);      \
+    set_lds_one(e, (const void *)k_ymul_n<T, NN, 1, true>, bytes);                    \
+    set_lds_one(e, (const void *)k_ymul_n<T, NN, fixed_cols(NN), true>, bytes);
     MMC_YL(64) MMC_YL(96) MMC_YL(128) MMC_YL(192) MMC_YL(256) MMC_YL(384) MMC_YL(512)
     MMC_YL(768) MMC_YL(1024) MMC_YL(2048)
 #undef MMC_YL
     set_lds_one(e, (const void *)k_yfwd_tensor<T>, bytes);
-    set_lds_one(e, (const void *)k_ymul_any<T, 0>, bytes);
-    set_lds_one(e, (const void *)k_ymul_any<T, 1>, bytes);
-    set_lds_one(e, (const void *)k_ymul_any<T, 2>, bytes);
-    set_lds_one(e, (const void *)k_ymul_any<T, 3>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 0, false>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 1, false>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 2, false>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 3, false>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 0, true>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 1, true>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 2, true>, bytes);
+    set_lds_one(e, (const void *)k_ymul_any<T, 3, true>, bytes);
 #define MMC_YZL(NN) set_lds_one(e, (const void *)k_yzconv_n<T, NN, 2>, bytes); \
     set_lds_one(e, (const void *)k_yzconv_n<T, NN, 3>, bytes);                 \
     set_lds_one(e, (const void *)k_yzconv_n<T, NN, 4>, bytes);
@@ -738,7 +759,7 @@ hipError_t set_lds_ypass(int bytes)
     template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int, int,     \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
-                                       cpx<T> *, const cpx<T> *, const cpx<T> *, size_t);     \
+                                       cpx<T> *, const cpx<T> *, const cpx<T> *, size_t, bool); \
     template hipError_t launch_yzconv<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                          cpx<T> *, const cpx<T> *, const cpx<T> *, size_t,     \
                                          bool);                                                \

@@ -40,6 +40,7 @@ struct Segment {
     int begin = 0, count = 0;
     size_t lds = 0;
     double bytes = 0;  // algorithmic HBM bytes of one launch of the segment
+    bool half = true;  // y pass: every instance's rows fill at most half the padded length
 };
 
 struct List {
@@ -523,6 +524,7 @@ void merge_segments(List &l, int pol)
         items.insert(items.end(), l.host.begin() + sg.begin, l.host.begin() + sg.begin + sg.count);
         m.lds = std::max(m.lds, sg.lds);
         m.bytes += sg.bytes;
+        m.half = m.half && sg.half;
     }
     m.count = (int)items.size();
     l.host.swap(items);
@@ -606,6 +608,7 @@ void build_lists(mmc_ctx *c)
                 Segment &sg = yclass[yk];
                 sg.fixed_n = fn;
                 sg.rows = yk.second;
+                sg.half = sg.half && 2 * in.ny <= in.Py;
                 sg.lds = std::max(sg.lds, ylds);
                 sg.bytes += inst_bytes(c, in, K_YMUL);
                 c->yl.lds = std::max(c->yl.lds, ylds);
@@ -1125,7 +1128,7 @@ int pass_zyz(mmc_ctx *c, bool sweep = false, bool a2 = false)
     if (rc) return rc;
     rc = run_segments(c, c->yl, K_YMUL, [&](const Segment &sg, hipStream_t s) {
         return launch_ymul<T>(s, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, b.B,
-                              b.Kh, b.tw, sg.lds);
+                              b.Kh, b.tw, sg.lds, sg.half);
     });
     if (rc) return rc;
     rc = run_segments(c, c->yzl, K_YZC, [&](const Segment &sg, hipStream_t s) {

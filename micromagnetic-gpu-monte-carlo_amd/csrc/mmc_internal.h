@@ -121,11 +121,13 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n,
 // y columns per item of the compile-time-planned z passes
 __host__ __device__ constexpr int z_cols(size_t tsize) { return tsize == 4 ? 64 : 32; }
 // fixed_n: the common y length of all items (compile-time-planned kernel),
-// or 0 for the generic kernel.
+// or 0 for the generic kernel; half: every item's instance has 2 ny <= Py,
+// so the planned transforms skip the zero upper half of their input and
+// the discarded upper half of their result.
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_n, int cols, cpx<T> *buf, const cpx<T> *khat,
-                       const cpx<T> *tw, size_t lds);
+                       const cpx<T> *tw, size_t lds, bool half);
 // Threads of a k_yzconv_n workgroup: its items hold all layers of one x
 // frequency (few large items), so each gets twice the usual threads.
 constexpr int kYzThreads = 512;
