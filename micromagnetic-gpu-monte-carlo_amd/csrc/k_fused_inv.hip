@@ -111,13 +111,12 @@ __device__ __forceinline__ void xinv_metro_tile(const Inst *__restrict__ I,
     const int per = H * npair;
     const Div dper(per), dnp(npair);
     if (!zf) {
-        // the spectrum as the y pass left it: y-contiguous in A, or for films
-        // whose y pass sums z directly (Inst::a2) pair-major in the B region,
-        // [3][nz][ny/2][Hx][2], where a row pair's spectrum is contiguous
-        // (whole lines instead of 16 bytes of each)
+        // the spectrum as the y pass left it: y-contiguous in A, or (Inst::a2)
+        // pair-major at pm_off, [3][nz][ny/2][Hx][2], where a row pair's
+        // spectrum is contiguous (whole lines instead of 16 bytes of each)
         const bool pm = in.a2;
         const long cstride = (long)H * nz * ny;
-        const cpx<T> *src = specb + (pm ? in.b_off : in.a_off);
+        const cpx<T> *src = specb + (pm ? in.pm_off : in.a_off);
         const int np2 = ny >> 1;
         const bool p16x = pm ? sizeof(T) == 4 : p16;
         // element e -> (component c, frequency k, pair q): pairs innermost

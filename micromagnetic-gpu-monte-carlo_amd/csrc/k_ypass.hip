@@ -244,7 +244,7 @@ __device__ __forceinline__ void kmul_conj(cpx<T> *px, cpx<T> *py, cpx<T> *pz, co
 template <typename T, int N, int C, bool HALF>
 __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T> *__restrict__ bufb,
                                            const cpx<T> *__restrict__ khatb,
-                                           const cpx<T> *__restrict__ twb)
+                                           const cpx<T> *__restrict__ twb, int a2)
 {
     // HALF: the instance's ny rows fit in the lower half of the padded
     // length (an open y axis, N = 2ny), so the forward transform reads only
@@ -329,6 +329,28 @@ __device__ __forceinline__ void ymul_fixed(const Inst &in, const Work &w, cpx<T>
     }
     __syncthreads();
     fft_fixed<T, N, ROWS, kThreads, false, false, OLO>(s, tw);
+    if (a2 && in.a2) {
+        // pair-major, for the inverse x pass of a fused sweep (Pz == 1):
+        // element (c, y) at P[((c*ny/2 + y/2)*Hx + kx)*2 + (y & 1)], the
+        // item's columns side by side
+        cpx<T> *P = bufb + in.pm_off;
+        const int np = Ly >> 1;
+        for (int e = threadIdx.x; e < 3 * np * C; e += kThreads) {
+            const int col = e % C, rp = e / C;
+            const int comp = rp / np, p = rp - comp * np;
+            if (col >= ncol) continue;
+            const int r = comp * C + col;
+            const cpx<T> a = cconj(s[r * N + 2 * p]), b = cconj(s[r * N + 2 * p + 1]);
+            cpx<T> *o = P + (((long)comp * np + p) * H + kx0 + col) * 2;
+            if (sizeof(T) == 4) {
+                store_pair16(o, a, b);
+            } else {
+                o[0] = a;
+                o[1] = b;
+            }
+        }
+        return;
+    }
     const Div dLy(Ly);
     for (int e = threadIdx.x; e < ROWS * Ly; e += kThreads) {
         const int r = dLy(e), y = e - r * Ly;
@@ -347,10 +369,10 @@ __global__ __launch_bounds__(kThreads) void k_ymul_n(const Inst *__restrict__ I,
                                                      const Work *__restrict__ W,
                                                      cpx<T> *__restrict__ bufb,
                                                      const cpx<T> *__restrict__ khatb,
-                                                     const cpx<T> *__restrict__ twb)
+                                                     const cpx<T> *__restrict__ twb, int a2)
 {
     const Work w = W[blockIdx.x];
-    ymul_fixed<T, N, C, HALF>(I[w.inst], w, bufb, khatb, twb);
+    ymul_fixed<T, N, C, HALF>(I[w.inst], w, bufb, khatb, twb, a2);
 }
 
 // One launch over y-pass items of every y length of a batch (any_y_len),
@@ -360,12 +382,12 @@ __global__ __launch_bounds__(kThreads) void k_ymul_any(const Inst *__restrict__ 
                                                        const Work *__restrict__ W,
                                                        cpx<T> *__restrict__ bufb,
                                                        const cpx<T> *__restrict__ khatb,
-                                                       const cpx<T> *__restrict__ twb)
+                                                       const cpx<T> *__restrict__ twb, int a2)
 {
     const Work w = W[blockIdx.x];
     const Inst &in = I[w.inst];
     switch (in.Py) {
-#define MMC_ANY(NN) case NN: ymul_fixed<T, NN, any_tile(POL, NN), HALF>(in, w, bufb, khatb, twb); break;
+#define MMC_ANY(NN) case NN: ymul_fixed<T, NN, any_tile(POL, NN), HALF>(in, w, bufb, khatb, twb, a2); break;
     MMC_ANY(64) MMC_ANY(96) MMC_ANY(128) MMC_ANY(192) MMC_ANY(256) MMC_ANY(384) MMC_ANY(512)
 #undef MMC_ANY
     }
@@ -503,7 +525,7 @@ void k_yzconv_n(const Inst *__restrict__ I,
         // (c, z, y) at B[(((c*NZ + z)*ny/2 + y/2)*Hx + kx)*2 + (y & 1)], so a
         // row pair's spectrum is contiguous; the items of 8 consecutive x
         // frequencies run on one XCD and fill each 128-byte line together
-        cpx<T> *A2 = Ab + in.b_off;
+        cpx<T> *A2 = Ab + in.pm_off;
         const int np = Ly >> 1;
         const Div dnp(np);
         for (int e = threadIdx.x; e < ROWS * np; e += NT) {
@@ -655,14 +677,15 @@ hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, int 
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_n,
                        int cols, cpx<T> *buf, const cpx<T> *khat, const cpx<T> *tw, size_t lds,
-                       bool half)
+                       bool half, bool a2_)
 {
+    const int a2 = a2_;
     if (n <= 0) return hipSuccess;
     const dim3 g(n), b(kThreads);
     if (fixed_n == kAnyLen) {
 #define MMC_YA(P)                                                                              \
-    if (half) MMC_LAUNCH((k_ymul_any<T, P, true>), g, b, lds, st, I, W, buf, khat, tw);        \
-    else MMC_LAUNCH((k_ymul_any<T, P, false>), g, b, lds, st, I, W, buf, khat, tw);
+    if (half) MMC_LAUNCH((k_ymul_any<T, P, true>), g, b, lds, st, I, W, buf, khat, tw, a2);        \
+    else MMC_LAUNCH((k_ymul_any<T, P, false>), g, b, lds, st, I, W, buf, khat, tw, a2);
         switch (cols) {
         case 0: MMC_YA(0) break;
         case 1: MMC_YA(1) break;
@@ -674,10 +697,10 @@ hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n, int 
     }
 #define MMC_YN(NN)                                                                              \
     if (fixed_n == NN) {                                                                        \
-        if (half && cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1, true>), g, b, lds, st, I, W, buf, khat, tw); \
-        else if (half) MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN), true>), g, b, lds, st, I, W, buf, khat, tw); \
-        else if (cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1, false>), g, b, lds, st, I, W, buf, khat, tw); \
-        else MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN), false>), g, b, lds, st, I, W, buf, khat, tw); \
+        if (half && cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1, true>), g, b, lds, st, I, W, buf, khat, tw, a2); \
+        else if (half) MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN), true>), g, b, lds, st, I, W, buf, khat, tw, a2); \
+        else if (cols == 1) MMC_LAUNCH((k_ymul_n<T, NN, 1, false>), g, b, lds, st, I, W, buf, khat, tw, a2); \
+        else MMC_LAUNCH((k_ymul_n<T, NN, fixed_cols(NN), false>), g, b, lds, st, I, W, buf, khat, tw, a2); \
         return post();                                                                          \
     }
     MMC_YN(64) MMC_YN(96) MMC_YN(128) MMC_YN(192) MMC_YN(256) MMC_YN(384) MMC_YN(512)
@@ -759,7 +782,8 @@ hipError_t set_lds_ypass(int bytes)
     template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int, int,     \
                                        const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
-                                       cpx<T> *, const cpx<T> *, const cpx<T> *, size_t, bool); \
+                                       cpx<T> *, const cpx<T> *, const cpx<T> *, size_t, bool, \
+                                       bool);                                                  \
     template hipError_t launch_yzconv<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                          cpx<T> *, const cpx<T> *, const cpx<T> *, size_t,     \
                                          bool);                                                \

@@ -708,6 +708,9 @@ void build_lists(mmc_ctx *c)
         c->yzl.seg.push_back(sg);
     }
     for (auto &kv : yitems) {
+        // pair-major stores of consecutive x frequencies fill 128-byte lines
+        // together when their items run on one XCD
+        if (kv.first.second > 0 && kv.first.second < 8) xcd_group(kv.second, 8 / kv.first.second);
         Segment sg = yclass[kv.first];
         sg.begin = c->yl.n();
         sg.count = (int)kv.second.size();
@@ -855,6 +858,14 @@ int alloc_all(mmc_ctx *c)
             } else {
                 in.b_off = -1;
             }
+            // a film without z passes whose y pass has a compile-time plan
+            // gets a region of its own for the pair-major spectrum
+            in.pm_off = -1;
+            if (in.Pz == 1 && !in.zd && (in.ny % 2) == 0 && fixed_fft_size(in.Py)) {
+                b = (b + 1) & ~(size_t)1;
+                in.pm_off = (long)b;
+                b += (size_t)3 * in.Hx * in.ny;
+            }
             in.khat_off = (long)kh;
             kh += (size_t)6 * in.Hx * in.Pz * in.Py;
             in.k_off = (long)k;
@@ -866,12 +877,15 @@ int alloc_all(mmc_ctx *c)
             in.twz = (long)tw; twiddles(in.Pz, tw32); twiddles(in.Pz, tw64); tw += in.Pz;
         }
     }
+    a = (a + 1) & ~(size_t)1;   // B regions start 16-byte aligned (FP32)
     for (auto &in : c->inst) {
         if (in.cal_demag) in.b_off = in.b_off < 0 ? in.a_off : (long)a + in.b_off;
         // the B region of a zd instance (no z transforms) is free; pair-major
         // layout of [3][nz][ny/2][Hx][2] fits it when ny is even (Pz > nz)
-        in.a2 = in.cal_demag && in.zd && (in.ny % 2) == 0 && in.b_off != in.a_off &&
-                (in.b_off % 2) == 0;
+        if (in.cal_demag && in.zd) in.pm_off = in.b_off != in.a_off ? in.b_off : -1;
+        else if (in.cal_demag && in.pm_off >= 0) in.pm_off += (long)a;
+        else in.pm_off = -1;
+        in.a2 = in.cal_demag && (in.ny % 2) == 0 && in.pm_off >= 0 && (in.pm_off % 2) == 0;
     }
     c->m_elems = m; c->a_elems = a; c->b_elems = b; c->khat_elems = kh; c->k_elems = k;
     c->ka_elems = ka;
@@ -1128,7 +1142,7 @@ int pass_zyz(mmc_ctx *c, bool sweep = false, bool a2 = false)
     if (rc) return rc;
     rc = run_segments(c, c->yl, K_YMUL, [&](const Segment &sg, hipStream_t s) {
         return launch_ymul<T>(s, c->dInst, c->yl.dev + sg.begin, sg.count, sg.fixed_n, sg.rows, b.B,
-                              b.Kh, b.tw, sg.lds, sg.half);
+                              b.Kh, b.tw, sg.lds, sg.half, a2);
     });
     if (rc) return rc;
     rc = run_segments(c, c->yzl, K_YZC, [&](const Segment &sg, hipStream_t s) {

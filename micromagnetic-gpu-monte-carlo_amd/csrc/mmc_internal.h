@@ -54,8 +54,9 @@ struct Inst {
     int zf;               // fused tiles span all layers and transform z too
     int zd;               // z convolution summed directly in the y pass
                           // (k_yzconv_n); Khat holds x-y spectra of z slices
-    int a2;               // zd, even ny: in fused sweeps the y pass writes the x
-                          // spectrum pair-major into the B region,
+    int a2;               // even ny, and zd or Pz == 1 with a compile-time y
+                          // plan: in fused sweeps the y pass writes the x
+                          // spectrum pair-major at pm_off,
                           // [3][nz][ny/2][Hx][2] (a row pair's spectrum
                           // contiguous), which the inverse x pass reads
     int kmode[6];         // Khat storage of each component: 0 zero, 1 real
@@ -63,6 +64,8 @@ struct Inst {
     long N, NP;
     long m_off, h_off;    // T
     long a_off, b_off;    // cpx<T>
+    long pm_off;          // cpx<T>, pair-major spectrum (a2): the B region of
+                          // a zd instance, a region of its own for Pz == 1
     long khat_off;        // cpx<T>
     long k_off;           // double
     long ka_off;          // cpx<double>, tensor-build scratch
@@ -124,10 +127,12 @@ __host__ __device__ constexpr int z_cols(size_t tsize) { return tsize == 4 ? 64 
 // or 0 for the generic kernel; half: every item's instance has 2 ny <= Py,
 // so the planned transforms skip the zero upper half of their input and
 // the discarded upper half of their result.
+// a2: in a fused sweep, instances with Inst::a2 store their field spectrum
+// pair-major at pm_off for the inverse x pass instead of back in place.
 template <typename T>
 hipError_t launch_ymul(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_n, int cols, cpx<T> *buf, const cpx<T> *khat,
-                       const cpx<T> *tw, size_t lds, bool half);
+                       const cpx<T> *tw, size_t lds, bool half, bool a2);
 // Threads of a k_yzconv_n workgroup: its items hold all layers of one x
 // frequency (few large items), so each gets twice the usual threads.
 constexpr int kYzThreads = 512;
