@@ -1,7 +1,8 @@
 """Pins of the FP64 oracle (CPU only).
 
 The reference ships no tests, golden vectors or fixtures, and its hot-path
-code cannot be built here, so each oracle function is pinned
+code cannot be built here (it needs the CUDA runtime and samples' headers
+and Boost), so each oracle function is pinned
 by something independent of its own text: published known-answer vectors,
 numpy's FFT, closed forms, a brute-force quadrature of the dipole kernel,
 symmetry and sum rules, and Boltzmann statistics.  Each check is chosen so a
