@@ -61,6 +61,20 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     V3<T> chi[CH];
     Nbrs<T> cnb[CH];
     Move<T> cmv[CH];
+    // Prefetching tiles of one layer (PFC, not folding z) load only the
+    // inputs that are not in the tile's rows: the rows y - 1 and y + 1
+    // (used when the cell's row is the tile's first or last, loaded either
+    // way so that no load sits under a lane condition) and the sum of the z
+    // neighbours (the one form the bond sums use); x neighbours and rows
+    // inside the tile are read from the rows in LDS (colour-0 cells, which
+    // this pass does not write).  Missing neighbours are zero, as in
+    // gather_nbrs.
+    const bool lrow = PFC > 0 && !zf;
+    V3<T> pym[CH], pyp[CH], pzs[CH];
+    const uint32_t unx = (uint32_t)nx, uny = (uint32_t)ny;
+    auto keep = [](bool ok, const V3<T> &v) -> V3<T> {
+        return {ok ? v.x : T(0), ok ? v.y : T(0), ok ? v.z : T(0)};
+    };
     auto fetch = [&](int q, int k) {
         const int e = threadIdx.x + q * kThreads;
         int zl, yy, j;
@@ -70,7 +84,20 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
         if (e < ncell && x < nx) {
             const long oh = ((long)z * ny + y) * hw + j;
             chi[k] = {hc[oh], hc[cst + oh], hc[2 * cst + oh]};
-            cnb[k] = gather_nbrs(in, m, x, y, z);
+            if (lrow) {
+                const int jm = nbr(y, -1, ny, in.pbc[1]), jp = nbr(y, +1, ny, in.pbc[1]);
+                pym[k] = keep(jm >= 0, load_m(m, (uint32_t)NN, cell_index(unx, uny, x, jm >= 0 ? jm : y, z)));
+                pyp[k] = keep(jp >= 0, load_m(m, (uint32_t)NN, cell_index(unx, uny, x, jp >= 0 ? jp : y, z)));
+                V3<T> zp = {T(0), T(0), T(0)}, zm = zp;
+                int jz;
+                if (nz > 1 && (jz = nbr(z, +1, nz, in.pbc[2])) >= 0)
+                    zp = load_m(m, (uint32_t)NN, cell_index(unx, uny, x, y, jz));
+                if (nz > 1 && (jz = nbr(z, -1, nz, in.pbc[2])) >= 0)
+                    zm = load_m(m, (uint32_t)NN, cell_index(unx, uny, x, y, jz));
+                pzs[k] = {zp.x + zm.x, zp.y + zm.y, zp.z + zm.z};
+            } else {
+                cnb[k] = gather_nbrs(in, m, x, y, z);
+            }
             cmv[k] = trial_move<T>(in, (uint32_t)((long)nx * (y + (long)ny * z) + x), sweep,
                                    stage);
         }
@@ -99,10 +126,6 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
     constexpr bool LNB = PFC == 0 && RZ == 1 && QC > 2 && QC <= 4;
     const bool lnb = LNB && !zf && nyt == 2;
     V3<T> ph[LNB ? QC : 1], po[LNB ? QC : 1], pz[LNB ? QC : 1];
-    auto keep = [](bool ok, const V3<T> &v) -> V3<T> {
-        return {ok ? v.x : T(0), ok ? v.y : T(0), ok ? v.z : T(0)};
-    };
-    const uint32_t unx = (uint32_t)nx, uny = (uint32_t)ny;
     if (LNB && lnb) {
         const auto rows = staged_load<3 * RZ * N, kThreads, cpx<T>>(3 * RZ * N, row_ld);
         const int jzp = nz > 1 ? nbr(z0, +1, nz, in.pbc[2]) : -1;
@@ -197,6 +220,22 @@ __device__ __forceinline__ void metro_xfwd_tile(const Inst *__restrict__ I,
                 const V3<T> mi = odd ? V3<T>{s[o].y, s[RZ * N + o].y, s[2 * RZ * N + o].y}
                                      : V3<T>{s[o].x, s[RZ * N + o].x, s[2 * RZ * N + o].x};
                 const long i = (long)nx * (y + (long)ny * z) + x;
+                if (lrow) {
+                    // tile row r (layer 0), column c
+                    auto row_m = [&](int r, int c) -> V3<T> {
+                        const int u = (r >> 1) * N + c;
+                        return (r & 1) ? V3<T>{s[u].y, s[RZ * N + u].y, s[2 * RZ * N + u].y}
+                                       : V3<T>{s[u].x, s[RZ * N + u].x, s[2 * RZ * N + u].x};
+                    };
+                    const int jp = nbr(x, +1, nx, in.pbc[0]), jm = nbr(x, -1, nx, in.pbc[0]);
+                    cnb[k].xp = keep(jp >= 0, row_m(yy, jp >= 0 ? jp : x));
+                    cnb[k].xm = keep(jm >= 0, row_m(yy, jm >= 0 ? jm : x));
+                    const V3<T> vm = row_m(yy > 0 ? yy - 1 : 0, x), vp = row_m(yy + 1 < nyt ? yy + 1 : yy, x);
+                    cnb[k].ym = yy > 0 ? vm : pym[k];
+                    cnb[k].yp = yy + 1 < nyt ? vp : pyp[k];
+                    cnb[k].zp = pzs[k];
+                    cnb[k].zm = {T(0), T(0), T(0)};
+                }
                 const V3<T> mn = metro_apply(in, m, i, mi, cnb[k], chi[k], cmv[k], bt);
                 if (odd) {
                     s[o].y = mn.x;
