@@ -246,10 +246,13 @@ def test_tiled_metropolis_matches_per_colour(mmc, geo, precision):
                                  dict(nx=64, ny=16, nz=16),
                                  dict(nx=128, ny=10, nz=16, pbc_z=1),
                                  dict(nx=64, ny=64, nz=1),
-                                 dict(nx=64, ny=63, nz=1)],
+                                 dict(nx=64, ny=63, nz=1),
+                                 dict(nx=64, ny=2, nz=1, pbc_y=1),
+                                 dict(nx=64, ny=6, nz=1, pbc_y=1)],
                          ids=["open", "pbc", "odd_open", "3d_open", "3d_pbcz", "wide_pbc",
                               "zfold_open", "zfold_pbc", "deep_z_pz32", "deep_z_pz16",
-                              "planned_y_pair_major", "planned_y_odd_rows"])
+                              "planned_y_pair_major", "planned_y_odd_rows",
+                              "two_periodic_rows", "six_periodic_rows"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_fused_sweeps_match_separate(mmc, geo, precision):
     """Inverse x pass fused with colour 0 and colour 1 fused with the next
