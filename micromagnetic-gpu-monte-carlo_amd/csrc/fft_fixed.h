@@ -218,10 +218,10 @@ __device__ __forceinline__ void fft_fixed(cpx<T> *s, const cpx<T> *__restrict__ 
 // Forward transform along the slow axis of a [N][ROWS] tile (element i of
 // row `row` at s[i*ROWS + row]): ROWS rows side by side, e.g. the y columns
 // of a z transform.  Same arithmetic as fft_fixed; synchronisation alike.
-template <typename T, int N, int ROWS, int NT = kThreads>
+template <typename T, int N, int ROWS, int NT = kThreads, bool ZIN = false, bool OLO = false>
 __device__ __forceinline__ void fft_fixed_cols(cpx<T> *s, const cpx<T> *__restrict__ tw)
 {
-    FixedStages<T, N, ROWS, 1, NT, false, ROWS, 1, false, false, typename FixedPlan<N>::type>::run(s, tw);
+    FixedStages<T, N, ROWS, 1, NT, false, ROWS, 1, ZIN, OLO, typename FixedPlan<N>::type>::run(s, tw);
 }
 
 // z lengths with a compile-time-planned z pass (3-D films of 8-32 layers,
@@ -232,7 +232,8 @@ __host__ __device__ constexpr bool fixed_z_size(int n) { return n == 16 || n == 
 // half of its result (fft_fixed's OLO).
 __host__ __device__ constexpr bool fixed_olo_ok(int n)
 {
-    return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
+    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 ||
+           n == 2048;
 }
 
 // Columns per y-pass workgroup of the fixed path: about 3072 complex per tile.

@@ -106,7 +106,10 @@ __global__ __launch_bounds__(kThreads) void k_zinv(const Inst *__restrict__ I,
 // in LDS (z outer, y contiguous), loaded and stored with y along the lanes,
 // transformed along z with neighbouring lanes on neighbouring columns.
 // Same arithmetic as the runtime-plan path (same radix order and twiddles).
-template <typename T, int PZ>
+// HALF: every item's instance has 2 nz <= PZ (an open z axis), so the
+// forward transform reads only the lower half of the z rows (the upper half
+// is padding) and the inverse forms only the nz rows it stores.
+template <typename T, int PZ, bool HALF>
 __global__ __launch_bounds__(kThreads) void k_zfwd_n(const Inst *__restrict__ I,
                                                      const Work *__restrict__ W,
                                                      const cpx<T> *__restrict__ srcb,
@@ -124,8 +127,9 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_n(const Inst *__restrict__ I,
     cpx<T> *dst = dstb + in.b_off + ((long)c * H + kx) * PZ * Ly + y0;
     const int nyt = min(TY, Ly - y0);
     const auto twr = load_twiddles<T, PZ>(twb + in.twz);
-    staged_copy<PZ * TY, kThreads, cpx<T>>(
-        PZ * TY,
+    constexpr int ZL = HALF ? PZ / 2 : PZ;   // z rows staged
+    staged_copy<ZL * TY, kThreads, cpx<T>>(
+        ZL * TY,
         [&](int e) {
             const int z = e / TY, y = e - z * TY;
             T vx = T(0), vy = T(0);
@@ -139,14 +143,14 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_n(const Inst *__restrict__ I,
         [&](int e, cpx<T> v) { s[e] = v; });
     store_twiddles<T, PZ>(tw, twr);
     __syncthreads();
-    fft_fixed_cols<T, PZ, TY>(s, tw);
+    fft_fixed_cols<T, PZ, TY, kThreads, HALF>(s, tw);
     for (int e = threadIdx.x; e < PZ * TY; e += kThreads) {
         const int z = e / TY, y = e - z * TY;
         if (y < nyt) dst[(long)z * Ly + y] = s[e];
     }
 }
 
-template <typename T, int PZ>
+template <typename T, int PZ, bool HALF>
 __global__ __launch_bounds__(kThreads) void k_zinv_n(const Inst *__restrict__ I,
                                                      const Work *__restrict__ W,
                                                      const cpx<T> *__restrict__ srcb,
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(kThreads) void k_zinv_n(const Inst *__restrict__ I,
         [&](int e, cpx<T> v) { s[e] = cconj(v); });
     store_twiddles<T, PZ>(tw, twr);
     __syncthreads();
-    fft_fixed_cols<T, PZ, TY>(s, tw);
+    fft_fixed_cols<T, PZ, TY, kThreads, false, HALF && fixed_olo_ok(PZ)>(s, tw);
     for (int e = threadIdx.x; e < Lz * TY; e += kThreads) {
         const int z = e / TY, y = e - z * TY;
         if (y < nyt) dst[(long)z * Ly + y] = cconj(s[e]);
@@ -644,15 +648,20 @@ __global__ __launch_bounds__(kThreads) void k_yfwd_tensor(const Inst *__restrict
 
 template <typename T>
 hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n, int mode,
-                       int fixed_pz, const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
+                       int fixed_pz, const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds,
+                       bool half)
 {
     if (n <= 0) return hipSuccess;
     if (mode == 0 && fixed_z_size(fixed_pz)) {
         const dim3 g(n), b(kThreads);
-        if (fixed_pz == 16) MMC_LAUNCH((k_zfwd_n<T, 16>), g, b, lds, st, I, W, src, dst, tw);
-        else if (fixed_pz == 32) MMC_LAUNCH((k_zfwd_n<T, 32>), g, b, lds, st, I, W, src, dst, tw);
-        else MMC_LAUNCH((k_zfwd_n<T, 64>), g, b, lds, st, I, W, src, dst, tw);
-        return post();
+#define MMC_ZF(PZ)                                                                             \
+    if (fixed_pz == PZ) {                                                                      \
+        if (half) MMC_LAUNCH((k_zfwd_n<T, PZ, true>), g, b, lds, st, I, W, src, dst, tw);     \
+        else MMC_LAUNCH((k_zfwd_n<T, PZ, false>), g, b, lds, st, I, W, src, dst, tw);         \
+        return post();                                                                         \
+    }
+        MMC_ZF(16) MMC_ZF(32) MMC_ZF(64)
+#undef MMC_ZF
     }
     MMC_LAUNCH(k_zfwd<T>, dim3(n), dim3(kThreads), lds, st, I, W, mode, src, dst, tw);
     return post();
@@ -660,15 +669,19 @@ hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n, int 
 
 template <typename T>
 hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n, int fixed_pz,
-                       const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds)
+                       const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw, size_t lds, bool half)
 {
     if (n <= 0) return hipSuccess;
     if (fixed_z_size(fixed_pz)) {
         const dim3 g(n), b(kThreads);
-        if (fixed_pz == 16) MMC_LAUNCH((k_zinv_n<T, 16>), g, b, lds, st, I, W, src, dst, tw);
-        else if (fixed_pz == 32) MMC_LAUNCH((k_zinv_n<T, 32>), g, b, lds, st, I, W, src, dst, tw);
-        else MMC_LAUNCH((k_zinv_n<T, 64>), g, b, lds, st, I, W, src, dst, tw);
-        return post();
+#define MMC_ZI(PZ)                                                                             \
+    if (fixed_pz == PZ) {                                                                      \
+        if (half) MMC_LAUNCH((k_zinv_n<T, PZ, true>), g, b, lds, st, I, W, src, dst, tw);     \
+        else MMC_LAUNCH((k_zinv_n<T, PZ, false>), g, b, lds, st, I, W, src, dst, tw);         \
+        return post();                                                                         \
+    }
+        MMC_ZI(16) MMC_ZI(32) MMC_ZI(64)
+#undef MMC_ZI
     }
     MMC_LAUNCH(k_zinv<T>, dim3(n), dim3(kThreads), lds, st, I, W, src, dst, tw);
     return post();
@@ -746,8 +759,10 @@ hipError_t set_lds_ypass(int bytes)
     hipError_t e = hipSuccess;
     set_lds_one(e, (const void *)k_zfwd<T>, bytes);
     set_lds_one(e, (const void *)k_zinv<T>, bytes);
-#define MMC_ZL(PZ) set_lds_one(e, (const void *)k_zfwd_n<T, PZ>, bytes); \
-    set_lds_one(e, (const void *)k_zinv_n<T, PZ>, bytes);
+#define MMC_ZL(PZ) set_lds_one(e, (const void *)k_zfwd_n<T, PZ, false>, bytes); \
+    set_lds_one(e, (const void *)k_zinv_n<T, PZ, false>, bytes);                  \
+    set_lds_one(e, (const void *)k_zfwd_n<T, PZ, true>, bytes);                   \
+    set_lds_one(e, (const void *)k_zinv_n<T, PZ, true>, bytes);
     MMC_ZL(16) MMC_ZL(32) MMC_ZL(64)
 #undef MMC_ZL
     set_lds_one(e, (const void *)k_ymul<T>, bytes);
@@ -778,9 +793,9 @@ hipError_t set_lds_ypass(int bytes)
 
 #define MMC_INST(T)                                                                           \
     template hipError_t launch_zfwd<T>(hipStream_t, const Inst *, const Work *, int, int, int, \
-                                       const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
+                                       const cpx<T> *, cpx<T> *, const cpx<T> *, size_t, bool); \
     template hipError_t launch_zinv<T>(hipStream_t, const Inst *, const Work *, int, int,     \
-                                       const cpx<T> *, cpx<T> *, const cpx<T> *, size_t);     \
+                                       const cpx<T> *, cpx<T> *, const cpx<T> *, size_t, bool); \
     template hipError_t launch_ymul<T>(hipStream_t, const Inst *, const Work *, int, int, int,\
                                        cpx<T> *, const cpx<T> *, const cpx<T> *, size_t, bool, \
                                        bool);                                                  \

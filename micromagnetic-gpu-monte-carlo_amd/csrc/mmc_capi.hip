@@ -136,6 +136,8 @@ struct mmc_ctx {
     bool any_ok = false;              // every instance admits those launches
     int z_fixed = 0;                  // common compile-time-planned z length of the
                                       // z passes (0: runtime plan)
+    bool z_half = false;              // planned z passes: every instance's z rows fill
+                                      // at most half its padded length (open z)
     bool concurrent = false;          // per-length launches of a pass on side streams
     std::vector<hipStream_t> side;    // side streams (default priority) and the
     hipEvent_t fork_ev = nullptr;     // events that fork them from / join them
@@ -559,6 +561,9 @@ void build_lists(mmc_ctx *c)
     for (const Inst &in : c->inst)
         if (in.cal_demag && in.Pz > 1 && in.Pz <= kZDirect && !in.zd) c->z_fixed = -1;
     if (c->z_fixed < 0) c->z_fixed = 0;
+    c->z_half = c->z_fixed > 0;
+    for (const Inst &in : c->inst)
+        if (in.cal_demag && in.Pz > kZDirect && !in.zd) c->z_half = c->z_half && 2 * in.nz <= in.Pz;
     for (int i = 0; i < c->count; ++i) {
         const Inst &in = c->inst[i];
         const bool zdirect = in.Pz <= kZDirect;
@@ -1123,7 +1128,7 @@ int pass_zfwd(mmc_ctx *c, bool sweep)
     const Bufs<T> b(c);
     const List &l = sweep ? c->zls : c->zl;
     if (l.n())
-        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(c->stream, c->dInst, l.dev, l.n(), 0, c->z_fixed, b.A, b.B, b.tw, l.lds); },
+        HIPCHK(c, timed(c, K_ZFWD, [&] { return launch_zfwd<T>(c->stream, c->dInst, l.dev, l.n(), 0, c->z_fixed, b.A, b.B, b.tw, l.lds, c->z_half); },
                         zbytes(c, sweep)));
     return MMC_OK;
 }
@@ -1151,7 +1156,7 @@ int pass_zyz(mmc_ctx *c, bool sweep = false, bool a2 = false)
     });
     if (rc) return rc;
     if (l.n())
-        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, l.dev, l.n(), c->z_fixed, b.B, b.A, b.tw, l.lds); },
+        HIPCHK(c, timed(c, K_ZINV, [&] { return launch_zinv<T>(st, c->dInst, l.dev, l.n(), c->z_fixed, b.B, b.A, b.tw, l.lds, c->z_half); },
                         zbytes(c, sweep)));
     return MMC_OK;
 }
@@ -1396,7 +1401,7 @@ int tensor_fft(mmc_ctx *c)
     int rc = MMC_OK;
     e = launch_xfwd<double>(st, c->dInst, c->txl.dev, c->txl.n(), 1, 0, 0, c->d_K, KA, tw, c->txl.lds);
     if (e == hipSuccess && c->tzl.n())
-        e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, 0, KA, KA, tw, c->tzl.lds);
+        e = launch_zfwd<double>(st, c->dInst, c->tzl.dev, c->tzl.n(), 1, 0, KA, KA, tw, c->tzl.lds, false);
     if (e == hipSuccess)
         e = launch_yfwd_tensor<double>(st, c->dInst, c->tyl.dev, c->tyl.n(), KA, KH, tw, c->tyl.lds);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -113,14 +113,16 @@ hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n,
                        cpx<T> *spec, const cpx<T> *tw, size_t lds);
 // fixed_pz: the common z length of all items when it has a compile-time
 // plan (16, 32, 64; magnetisation mode, y tiles of kZCols columns), else 0.
+// half (planned passes only): every item's instance has 2 nz <= Pz, so the
+// transforms skip the zero upper half of their input / output.
 template <typename T>
 hipError_t launch_zfwd(hipStream_t st, const Inst *I, const Work *W, int n,
                        int mode, int fixed_pz, const cpx<T> *src, cpx<T> *dst,
-                       const cpx<T> *tw, size_t lds);
+                       const cpx<T> *tw, size_t lds, bool half);
 template <typename T>
 hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_pz, const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw,
-                       size_t lds);
+                       size_t lds, bool half);
 // y columns per item of the compile-time-planned z passes
 __host__ __device__ constexpr int z_cols(size_t tsize) { return tsize == 4 ? 64 : 32; }
 // fixed_n: the common y length of all items (compile-time-planned kernel),
