@@ -112,7 +112,7 @@ hipError_t launch_xfwd(hipStream_t st, const Inst *I, const Work *W, int n,
                        int mode, int fixed_n, int rows, const T *real,
                        cpx<T> *spec, const cpx<T> *tw, size_t lds);
 // fixed_pz: the common z length of all items when it has a compile-time
-// plan (16, 32, 64; magnetisation mode, y tiles of kZCols columns), else 0.
+// plan (16, 32, 64; magnetisation mode, y tiles of z_cols columns), else 0.
 // half (planned passes only): every item's instance has 2 nz <= Pz, so the
 // transforms skip the zero upper half of their input / output.
 template <typename T>
@@ -123,8 +123,10 @@ template <typename T>
 hipError_t launch_zinv(hipStream_t st, const Inst *I, const Work *W, int n,
                        int fixed_pz, const cpx<T> *src, cpx<T> *dst, const cpx<T> *tw,
                        size_t lds, bool half);
-// y columns per item of the compile-time-planned z passes
-__host__ __device__ constexpr int z_cols(size_t tsize) { return tsize == 4 ? 64 : 32; }
+// y columns per item of the compile-time-planned z passes: 32 (256 bytes of
+// each z row in FP32), one radix-4 butterfly per thread in a z stage of
+// length 32; 64 columns (half the workgroups) and 16 were both slower
+__host__ __device__ constexpr int z_cols(size_t) { return 32; }
 // fixed_n: the common y length of all items (compile-time-planned kernel),
 // or 0 for the generic kernel; half: every item's instance has 2 ny <= Py,
 // so the planned transforms skip the zero upper half of their input and

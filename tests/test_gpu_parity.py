@@ -24,6 +24,7 @@ CASES = [
     ("two_layer_3d_zconv", dict(nx=32, ny=32, nz=2)),
     ("deep_z_3d", dict(nx=16, ny=16, nz=16)),                # Pz = 32, compile-time z plan
     ("deep_z_pbcz_3d", dict(nx=12, ny=10, nz=16, pbc_z=1)),  # Pz = 16, compile-time z plan
+    ("deep_z_ragged_y_3d", dict(nx=8, ny=40, nz=16)),        # z tiles of 32 + 8 y columns
 ]
 
 
