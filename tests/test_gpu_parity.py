@@ -168,6 +168,32 @@ def test_determinism_and_batch(orc, mmc):
         assert np.array_equal(b2.m(i), b.m(i))
 
 
+@pytest.mark.parametrize("precision", [1, 0], ids=["fp32", "fp64"])
+def test_batch_mixed_z_padding(mmc, precision):
+    """Two 3-D films sharing the planned z length 32: an open one (nz = 16,
+    whose z transforms alone skip the zero half) and a periodic one (nz =
+    32).  Batched, the z passes run unpruned for both; each instance must
+    still match its own single context bit for bit."""
+    geos = [dict(FILM3D, nx=12, ny=10, nz=16, seed=5),
+            dict(FILM3D, nx=8, ny=12, nz=32, pbc_z=1, seed=6)]
+    singles = []
+    for kw in geos:
+        c = mmc.Context(mmc.make_params(**kw), precision)
+        c.build_tensor()
+        c.init_random_m()
+        c.set_temperature(600.0)
+        c.sweeps(0, 0, 4)
+        singles.append((c.m(), c.field()))
+    b = mmc.Context([mmc.make_params(**kw) for kw in geos], precision)
+    b.build_tensor()
+    b.init_random_m()
+    b.set_temperature(600.0)
+    b.sweeps(0, 0, 4)
+    for i, (m, h) in enumerate(singles):
+        assert np.array_equal(b.m(i), m), f"instance {i}"
+        assert np.array_equal(b.field(inst=i), h), f"instance {i} field"
+
+
 def _run(mmc, kw, precision, n, graph=False, tyx=None, ncol=None, chunk=None):
     c = mmc.Context(mmc.make_params(**kw), precision)
     c.build_tensor()
